@@ -1,0 +1,66 @@
+// Internal launcher interface shared by the kernels and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace simd2 {
+
+// Row-major problem every kernel solves:
+//   O[r][c] = (+)_{k<K} X[r][k] (x) Y[k][c]   (r < R, c < Cn)
+// then, per element, the epilogue below.  The column-major C ABI maps onto
+// it by swapping operands (a column-major MxN product is the row-major NxM
+// product of the transposes), which is exact because every (x) here is
+// commutative bit for bit.
+struct Epilogue {
+  const float *C = nullptr;  // O = O (+) C when set (may alias O)
+  int64_t ldc = 0;
+  const float *P = nullptr;  // previous values: change detection when set
+  int64_t ldp = 0;
+  int *flag = nullptr;       // set to 1 when any element changed
+  int *tile_flags = nullptr; // per output tile: 1 changed / 0 unchanged
+  int mode = 0;              // change test: 0 bits differ, 1 |p-o| > 1e-4,
+                             // 2 APLP: o in (-1,1) -> -FLT_MAX, then o != p
+};
+
+enum CheckMode { CHECK_EXACT = 0, CHECK_THRESHOLD = 1, CHECK_APLP = 2 };
+
+// Tile configurations of the VALU kernel.
+enum ValuCfg { CFG_AUTO = -1, CFG_L128 = 0, CFG_S64 = 1, CFG_COUNT = 2 };
+struct ValuCfgInfo { int bm, bn, bk, threads; };
+ValuCfgInfo valu_cfg_info(int cfg);
+int valu_pick_cfg(int R, int Cn, int K);
+
+hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,
+                           int64_t ldx, const float *Y, int64_t ldy, float *O,
+                           int64_t ldo, const Epilogue &e, hipStream_t s,
+                           int cfg = CFG_AUTO);
+
+// plus_mul on MFMA (v_mfma_f32_16x16x4_f32), same row-major problem.
+hipError_t launch_plusmul_mfma(int R, int Cn, int K, const float *X,
+                               int64_t ldx, const float *Y, int64_t ldy,
+                               float *O, int64_t ldo, const float *C,
+                               int64_t ldc, hipStream_t s);
+
+// or_and on int32 booleans (row-major problem as above).
+hipError_t launch_orand_i32(int R, int Cn, int K, const int *X, int64_t ldx,
+                            const int *Y, int64_t ldy, int *O, int64_t ldo,
+                            const int *C, int64_t ldc, const int *P,
+                            int64_t ldp, int *flag, hipStream_t s);
+
+// or_and on bit-packed rows.
+hipError_t launch_orand_packed(int M, int N, int K, const uint32_t *A,
+                               int64_t lda, const uint32_t *B, int64_t ldb,
+                               const uint32_t *C, int64_t ldc, uint32_t *D,
+                               int64_t ldd, const uint32_t *P, int64_t ldp,
+                               int *flag, hipStream_t s);
+
+// Element-wise helpers (converge.hip).
+hipError_t launch_comp_update_f32(float *prev, float *cur, int *flag,
+                                  int64_t n, int mode, hipStream_t s);
+hipError_t launch_comp_update_i32(int *prev, const int *cur, int *flag,
+                                  int64_t n, hipStream_t s);
+hipError_t launch_fill_i32(int *p, int v, int64_t n, hipStream_t s);
+hipError_t launch_copy_2d(void *dst, int64_t ldd, const void *src,
+                          int64_t lds, int rows, int row_bytes, hipStream_t s);
+
+}  // namespace simd2

@@ -1,0 +1,240 @@
+// or_and semiring on gfx950: D = [C |] OR_k A(i,k) & B(k,j).
+//
+// Packed kernel (simd2_orand_packed): rows of A are K-bit vectors, rows of
+// B and D are N-bit vectors, 32 columns per word.  Row i of D is the OR of
+// the rows k of B for which bit A(i,k) is set.  A thread owns TR rows x TW
+// words of D; per k it turns its rows' A bits into all-ones / zero masks with
+// one v_bfe_i32 each and folds B's words in with one v_and_or_b32 per word,
+// i.e. 32 (i,j,k) terms per instruction.  A B slab of 32 rows (one A word per
+// row of the tile) sits in LDS; A words come straight from global memory.
+// A whole A word that is zero skips its 32 k-steps (wave-uniform test), which
+// pays off on sparse graphs.
+//
+// int32 kernel (cuasr_orand_srsgemm drop-in): booleans stored one per int.
+// The kernel packs on the fly: X rows (k contiguous) are packed along k with
+// wave ballots, Y columns are packed along k by each lane walking its own
+// column, and a thread then ORs the AND of 32-k words per output element
+// (v_and_or_b32 again, 32 terms per instruction).  Its cost is the 4-byte
+// reads of 1-bit data, which is why the packed form exists.
+#include "internal.hpp"
+
+namespace simd2 {
+
+namespace {
+
+// acc | (a & b) as one instruction; written as asm because the compiler
+// otherwise pairs terms into v_and + v_or3 (1.5 instructions per term).
+__device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t b, uint32_t acc) {
+  uint32_t r;
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(acc));
+  return r;
+}
+
+constexpr int PK_TR = 4;     // rows per thread
+constexpr int PK_TW = 4;     // words per thread
+constexpr int PK_NTR = 16;   // threads along rows
+constexpr int PK_NTW = 16;   // threads along words
+constexpr int PK_BM = PK_TR * PK_NTR;   // 64 rows per tile
+constexpr int PK_BW = PK_TW * PK_NTW;   // 64 words (2048 columns) per tile
+constexpr int PK_BK = 32;               // k per slab (one A word)
+
+__global__ __launch_bounds__(256) void orand_packed_kernel(
+    int M, int NW, int K, const uint32_t *__restrict__ A, int64_t lda,
+    const uint32_t *__restrict__ B, int64_t ldb, const uint32_t *C,
+    int64_t ldc, uint32_t *D, int64_t ldd, const uint32_t *P, int64_t ldp,
+    int *flag, uint32_t last_mask, int tiles_w) {
+  __shared__ __attribute__((aligned(16))) uint32_t Bs[PK_BK][PK_BW];
+  const int tid = threadIdx.x;
+  const int tw = tid % PK_NTW;
+  const int trr = tid / PK_NTW;
+  const int tile_r = blockIdx.x / tiles_w;
+  const int tile_w = blockIdx.x % tiles_w;
+  const int row0 = tile_r * PK_BM + trr * PK_TR;
+  const int w0 = tile_w * PK_BW;
+
+  uint32_t acc[PK_TR][PK_TW];
+#pragma unroll
+  for (int i = 0; i < PK_TR; ++i)
+#pragma unroll
+    for (int j = 0; j < PK_TW; ++j) acc[i][j] = 0u;
+
+  const int nkw = (K + 31) / 32;
+  for (int kw = 0; kw < nkw; ++kw) {
+    // A words of this thread's rows for k in [32 kw, 32 kw + 32).
+    uint32_t aw[PK_TR];
+    uint32_t any = 0;
+#pragma unroll
+    for (int i = 0; i < PK_TR; ++i) {
+      const int r = row0 + i;
+      aw[i] = (r < M) ? A[(int64_t)r * lda + kw] : 0u;
+      if (kw == nkw - 1 && (K & 31)) aw[i] &= (1u << (K & 31)) - 1u;
+      any |= aw[i];
+    }
+    const bool tile_any = __syncthreads_or(any != 0);
+    if (!tile_any) continue;  // whole slab contributes nothing
+    // Stage B rows k in the slab, words [w0, w0 + PK_BW).
+    for (int f = tid; f < PK_BK * PK_BW; f += 256) {
+      const int kk = f / PK_BW, ww = f % PK_BW;
+      const int k = kw * 32 + kk, w = w0 + ww;
+      Bs[kk][ww] = (k < K && w < NW) ? B[(int64_t)k * ldb + w] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < PK_BK; ++kk) {
+      uint32_t bw[PK_TW];
+#pragma unroll
+      for (int j = 0; j < PK_TW; ++j) bw[j] = Bs[kk][tw * PK_TW + j];
+#pragma unroll
+      for (int i = 0; i < PK_TR; ++i) {
+        // bit kk of the A word as an all-ones / all-zeros mask (v_bfe_i32)
+        const uint32_t m = (uint32_t)((int32_t)(aw[i] << (31 - kk)) >> 31);
+#pragma unroll
+        for (int j = 0; j < PK_TW; ++j) acc[i][j] = and_or(m, bw[j], acc[i][j]);
+      }
+    }
+    __syncthreads();
+  }
+
+  bool changed = false;
+#pragma unroll
+  for (int i = 0; i < PK_TR; ++i) {
+    const int r = row0 + i;
+    if (r >= M) continue;
+#pragma unroll
+    for (int j = 0; j < PK_TW; ++j) {
+      const int w = w0 + tw * PK_TW + j;
+      if (w >= NW) continue;
+      uint32_t v = acc[i][j];
+      if (C) v |= C[(int64_t)r * ldc + w];
+      if (w == NW - 1) v &= last_mask;
+      if (P) changed |= (v != (P[(int64_t)r * ldp + w] & (w == NW - 1 ? last_mask : ~0u)));
+      D[(int64_t)r * ldd + w] = v;
+    }
+  }
+  if (flag) {
+    const bool any = __any(changed);
+    if (any && (tid % warpSize) == 0) atomicOr(flag, 1);
+  }
+}
+
+// ---- int32 booleans -------------------------------------------------------
+constexpr int IB_BM = 64;   // output rows per tile
+constexpr int IB_BN = 64;   // output columns per tile
+constexpr int IB_BKW = 8;   // 32-bit k-words per slab (256 k)
+
+__global__ __launch_bounds__(256) void orand_i32_kernel(
+    int R, int Cn, int K, const int *__restrict__ X, int64_t ldx,
+    const int *__restrict__ Y, int64_t ldy, const int *C, int64_t ldc,
+    int *O, int64_t ldo, const int *P, int64_t ldp, int *flag, int tiles_c) {
+  __shared__ uint32_t Xp[IB_BM][IB_BKW + 1];   // rows packed along k
+  __shared__ uint32_t Yp[IB_BN][IB_BKW + 1];   // columns packed along k
+  const int tid = threadIdx.x;
+  const int lane = tid % 64;
+  const int wave = tid / 64;
+  const int row0 = (blockIdx.x / tiles_c) * IB_BM;
+  const int col0 = (blockIdx.x % tiles_c) * IB_BN;
+  const int tr = tid / 16;  // 16 x 16 threads, 4 x 4 outputs each
+  const int tc = tid % 16;
+
+  uint32_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0u;
+
+  for (int k0 = 0; k0 < K; k0 += IB_BKW * 32) {
+    // X: each wave packs 16 rows; per row, 4 ballots of 64 consecutive k.
+    for (int rr = 0; rr < IB_BM / 4; ++rr) {
+      const int lr = wave * (IB_BM / 4) + rr;
+      const int r = row0 + lr;
+#pragma unroll
+      for (int ch = 0; ch < IB_BKW / 2; ++ch) {
+        const int k = k0 + ch * 64 + lane;
+        const bool bit = (r < R && k < K) ? (X[(int64_t)r * ldx + k] != 0) : false;
+        const uint64_t m = __ballot(bit);
+        if (lane == 0) {
+          Xp[lr][ch * 2] = (uint32_t)m;
+          Xp[lr][ch * 2 + 1] = (uint32_t)(m >> 32);
+        }
+      }
+    }
+    // Y: lane = column, each wave walks 64 k (two words) of its quarter.
+    {
+      const int c = col0 + lane;
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        uint32_t wv = 0;
+        const int kb = k0 + wave * 64 + half * 32;
+        for (int q = 0; q < 32; ++q) {
+          const int k = kb + q;
+          const bool bit = (c < Cn && k < K) ? (Y[(int64_t)k * ldy + c] != 0) : false;
+          wv |= (uint32_t)bit << q;
+        }
+        Yp[lane][wave * 2 + half] = wv;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < IB_BKW; ++w) {
+      uint32_t xa[4], yb[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xa[i] = Xp[tr * 4 + i][w];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) yb[j] = Yp[tc * 4 + j][w];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = and_or(xa[i], yb[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+
+  bool changed = false;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = row0 + tr * 4 + i;
+    if (r >= R) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = col0 + tc * 4 + j;
+      if (c >= Cn) continue;
+      int v = acc[i][j] != 0u;
+      if (C) v = v || (C[(int64_t)r * ldc + c] != 0);
+      if (P) changed |= (v != P[(int64_t)r * ldp + c]);
+      O[(int64_t)r * ldo + c] = v;
+    }
+  }
+  if (flag) {
+    const bool any = __any(changed);
+    if (any && (tid % warpSize) == 0) atomicOr(flag, 1);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_orand_packed(int M, int N, int K, const uint32_t *A,
+                               int64_t lda, const uint32_t *B, int64_t ldb,
+                               const uint32_t *C, int64_t ldc, uint32_t *D,
+                               int64_t ldd, const uint32_t *P, int64_t ldp,
+                               int *flag, hipStream_t s) {
+  const int NW = (N + 31) / 32;
+  const uint32_t last_mask = (N & 31) ? ((1u << (N & 31)) - 1u) : ~0u;
+  const int tiles_r = (M + PK_BM - 1) / PK_BM;
+  const int tiles_w = (NW + PK_BW - 1) / PK_BW;
+  orand_packed_kernel<<<dim3(tiles_r * tiles_w), dim3(256), 0, s>>>(
+      M, NW, K, A, lda, B, ldb, C, ldc, D, ldd, P, ldp, flag, last_mask, tiles_w);
+  return hipGetLastError();
+}
+
+hipError_t launch_orand_i32(int R, int Cn, int K, const int *X, int64_t ldx,
+                            const int *Y, int64_t ldy, int *O, int64_t ldo,
+                            const int *C, int64_t ldc, const int *P,
+                            int64_t ldp, int *flag, hipStream_t s) {
+  const int tiles_r = (R + IB_BM - 1) / IB_BM;
+  const int tiles_c = (Cn + IB_BN - 1) / IB_BN;
+  orand_i32_kernel<<<dim3(tiles_r * tiles_c), dim3(256), 0, s>>>(
+      R, Cn, K, X, ldx, Y, ldy, C, ldc, O, ldo, P, ldp, flag, tiles_c);
+  return hipGetLastError();
+}
+
+}  // namespace simd2

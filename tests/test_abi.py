@@ -1,0 +1,106 @@
+"""CPU tests of the C ABI: libsimd2.so loads, exports every function the
+headers declare, and rejects bad arguments before touching the GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+import simd2_amd
+
+HEADERS = [os.path.join(ROOT, "include", "simd2", h) for h in ("srgemm.h", "converge.h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:int|bool|char|void)\s*\*?\s*(\w+)\s*\(",
+                             text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def exported_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", simd2_amd.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(simd2_amd.LIB_PATH):
+        simd2_amd.build()
+    return simd2_amd.lib()
+
+
+def test_headers_declare_expected_surface():
+    names = declared_functions()
+    for n in simd2_amd.EXPORTS:
+        assert n in names, n
+    assert len(names) >= 19
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = declared_functions() - exported_symbols()
+    assert not missing, missing
+
+
+def test_status_strings(lib):
+    assert simd2_amd.status_string(0) == "success"
+    assert "overlaps" in simd2_amd.status_string(simd2_amd.ERROR_ALIASING)
+    assert simd2_amd.status_string(999) == "unknown status"
+
+
+# Fake, never-dereferenced device addresses: every call below must be
+# rejected by the host-side checks, before any kernel launch.
+P1, P2, P3, P4 = 1 << 20, 2 << 20, 3 << 20, 4 << 20
+
+
+@pytest.mark.parametrize("name", list(simd2_amd.DROPIN.values()))
+def test_dropin_argument_errors(lib, name):
+    fn = getattr(lib, name)
+    assert fn(-1, 4, 4, P1, 4, P2, 4, P3, 4, P4, True, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert fn(4, 4, 4, P1, 3, P2, 4, P3, 4, P4, True, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert fn(4, 4, 4, P1, 4, P2, 4, P3, 4, None, True, None) == simd2_amd.ERROR_NULL_POINTER
+    assert fn(4, 4, 4, None, 4, P2, 4, P3, 4, P4, True, None) == simd2_amd.ERROR_NULL_POINTER
+    assert fn(4, 4, 4, P1, 4, P2, 4, None, 4, P4, True, None) == simd2_amd.ERROR_NULL_POINTER
+    # D overlapping A (or B) is rejected; C == D is allowed
+    assert fn(4, 4, 4, P1, 4, P2, 4, P3, 4, P1 + 8, True, None) == simd2_amd.ERROR_ALIASING
+    assert fn(4, 4, 4, P1, 4, P2, 4, P3, 4, P2, True, None) == simd2_amd.ERROR_ALIASING
+    # empty outputs are a successful no-op
+    assert fn(0, 4, 4, P1, 4, P2, 4, P3, 4, P4, True, None) == simd2_amd.SUCCESS
+    assert fn(4, 0, 4, P1, 4, P2, 4, P3, 4, P4, True, None) == simd2_amd.SUCCESS
+
+
+def test_generic_entry_point_errors(lib):
+    f = lib.simd2_srgemm
+    assert f(9, 4, 4, 4, P1, 4, P2, 4, P3, 4, P4, 4, 1, None) == simd2_amd.ERROR_NOT_SUPPORTED
+    assert f(-1, 4, 4, 4, P1, 4, P2, 4, P3, 4, P4, 4, 1, None) == simd2_amd.ERROR_NOT_SUPPORTED
+    assert f(0, 4, 4, 4, P1, 4, P2, 4, P3, 4, P4, 3, 1, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    g = lib.simd2_orand_packed
+    assert g(4, 40, 4, P1, 1, P2, 1, P3, 2, P4, 2, 1, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert g(4, 40, 4, P1, 1, P2, 2, P3, 2, None, 2, 1, None) == simd2_amd.ERROR_NULL_POINTER
+    assert g(4, 40, 40, P1, 2, P2, 2, P3, 2, P1, 2, 1, None) == simd2_amd.ERROR_ALIASING
+
+
+def test_converge_argument_errors(lib):
+    itr = ctypes.c_int(-5)
+    f = lib.simd2_converge_f32
+    assert f(1, 8, P1, 8, P2, 8, P3, 1, 0, 10, 1, None, ctypes.byref(itr)) == \
+        simd2_amd.ERROR_NOT_SUPPORTED          # plus_mul has no fixpoint loop
+    assert itr.value == 0
+    assert f(0, 8, P1, 8, P2, 8, P3, 1, 7, 10, 1, None, ctypes.byref(itr)) == \
+        simd2_amd.ERROR_NOT_SUPPORTED          # unknown check mode
+    assert f(0, 8, P1, 7, P2, 8, P3, 1, 0, 10, 1, None, ctypes.byref(itr)) == \
+        simd2_amd.ERROR_INVALID_PROBLEM
+    assert f(0, 8, P1, 8, P1 + 16, 8, P3, 1, 0, 10, 1, None, ctypes.byref(itr)) == \
+        simd2_amd.ERROR_ALIASING
+    assert f(0, 8, None, 8, P2, 8, P3, 1, 0, 10, 1, None, ctypes.byref(itr)) == \
+        simd2_amd.ERROR_NULL_POINTER
+    assert f(0, 0, None, 0, None, 0, None, 1, 0, 10, 1, None, ctypes.byref(itr)) == \
+        simd2_amd.SUCCESS

@@ -1,0 +1,202 @@
+"""GPU parity of the semiring GEMM entry points against the oracle.
+
+Bit-exact for the six min/max semirings and or_and (and for plus_mul /
+minus_square where they take the sequential VALU path on shapes off the
+128x128x16 grid); plus_mul on MFMA and the fused minus_square within 1e-5
+relative of FP64 (tolerance stated per test)."""
+import sys
+
+import numpy as np
+import pytest
+
+from util import EXACT_OPS, OPS_F32, golden_cases, semiring_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bits(x):
+    return np.ascontiguousarray(x).view(np.uint32)
+
+
+def run(torch, op, A, B, C, epi):
+    import simd2_amd
+    dt = torch.int32 if op == "or_and" else torch.float32
+    tA = torch.from_numpy(np.ascontiguousarray(A)).to("cuda", dt)
+    tB = torch.from_numpy(np.ascontiguousarray(B)).to("cuda", dt)
+    tC = torch.from_numpy(np.ascontiguousarray(C)).to("cuda", dt)
+    D = simd2_amd.srgemm(op, tA, tB, tC, epi)
+    torch.cuda.synchronize()
+    return D.cpu().numpy()
+
+
+def fp64_ref(op, A, B, C, epi):
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    if op == "plus_mul":
+        R = A64 @ B64
+    else:
+        R = (A64 ** 2).sum(1)[:, None] + (B64 ** 2).sum(0)[None, :] - 2 * A64 @ B64
+    return R + C.astype(np.float64) if epi else R
+
+
+def test_golden_cases_all_ops(gpu, golden_srgemm):
+    """The reference test's inputs and the reference host_srgemm's outputs."""
+    g = golden_srgemm
+    for tag, M, N, K, epi in golden_cases(g):
+        A, B, C = g[f"{tag}/A"], g[f"{tag}/B"], g[f"{tag}/C"]
+        for op in OPS_F32:
+            say(f"golden {tag} {op} M={M} N={N} K={K} epi={epi}")
+            D = run(gpu, op, A, B, C, epi)
+            want = g[f"{tag}/{op}"]
+            if op == "minus_square":
+                # fused (a-b)^2 + acc: one rounding fewer than the oracle
+                np.testing.assert_allclose(D, want, rtol=1e-5, atol=1e-6)
+            else:  # includes plus_mul: these shapes take the sequential VALU path
+                assert np.array_equal(bits(D), bits(want)), (tag, op)
+        Ai, Bi, Ci = ((x.astype(np.int32) % 2).astype(np.int32) for x in (A, B, C))
+        say(f"golden {tag} or_and")
+        assert np.array_equal(run(gpu, "or_and", Ai, Bi, Ci, epi), g[f"{tag}/or_and"])
+
+
+def test_reference_test_shapes(gpu, oracle_mod):
+    """The 25 random shapes of tests/cuAsr_kernel_test.cu (glibc rand, dims
+    < 256, some on and off the tile grid) with holes and integer data."""
+    shapes = [(103, 198, 105), (205, 186, 171), (84, 248, 27), (99, 51, 159), (49, 88, 163),
+              (233, 94, 212), (84, 17, 14), (135, 112, 233), (62, 1, 126), (56, 92, 42),
+              (60, 84, 236), (67, 251, 250), (5, 60, 124), (249, 92, 187), (241, 179, 5),
+              (229, 202, 11), (31, 35, 30), (115, 90, 197), (100, 36, 17), (242, 27, 16),
+              (72, 71, 21), (155, 125, 245), (248, 41, 248), (232, 152, 50), (188, 95, 78)]
+    for t, (M, N, K) in enumerate(shapes):
+        for op in EXACT_OPS:
+            epi = bool(t % 2)
+            A, B, C = semiring_inputs(op, M, N, K, seed=100 + t)
+            say(f"shape {t} {op} {M}x{N}x{K} epi={epi}")
+            want = oracle_mod.srgemm_f32(op, A, B, C, epi, threads=8)
+            assert np.array_equal(bits(run(gpu, op, A, B, C, epi)), bits(want)), (t, op)
+
+
+@pytest.mark.parametrize("M,N,K", [(128, 128, 16), (256, 384, 512), (512, 512, 512),
+                                   (1024, 256, 64)])
+def test_tile_grid_exact_ops(gpu, oracle_mod, M, N, K):
+    """Shapes on the 128x128x16 grid (fast path, both tile configurations)."""
+    for op in EXACT_OPS:
+        for epi in (False, True):
+            A, B, C = semiring_inputs(op, M, N, K, seed=M + N + K)
+            say(f"grid {op} {M}x{N}x{K} epi={epi}")
+            want = oracle_mod.srgemm_f32(op, A, B, C, epi, threads=8)
+            assert np.array_equal(bits(run(gpu, op, A, B, C, epi)), bits(want)), op
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 256, 256), (512, 384, 1024), (200, 300, 100)])
+def test_plus_mul_and_minus_square_tolerance(gpu, M, N, K):
+    """plus_mul (MFMA on the grid, VALU off it) and minus_square (fused FMA)
+    within 1e-5 relative of FP64; inputs U{1..100} without holes."""
+    for op in ("plus_mul", "minus_square"):
+        for epi in (False, True):
+            A, B, C = semiring_inputs(op, M, N, K, seed=7)
+            say(f"tol {op} {M}x{N}x{K} epi={epi}")
+            D = run(gpu, op, A, B, C, epi).astype(np.float64)
+            ref = fp64_ref(op, A, B, C, epi)
+            rel = np.max(np.abs(D - ref) / np.abs(ref))
+            assert rel <= 1e-5, (op, rel)
+
+
+def test_plus_mul_mfma_fractional(gpu):
+    """MFMA layout check with non-integer data (an index or transpose slip in
+    the operand layout cannot pass)."""
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((256, 128)).astype(np.float32)
+    B = rng.standard_normal((128, 384)).astype(np.float32)
+    C = rng.standard_normal((256, 384)).astype(np.float32)
+    D = run(gpu, "plus_mul", A, B, C, True).astype(np.float64)
+    ref = A.astype(np.float64) @ B.astype(np.float64) + C
+    assert np.max(np.abs(D - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+def test_large_rows_sampled(gpu, oracle_mod):
+    """min_plus and max_min at 2048^3: rows sampled against the oracle."""
+    import torch
+    import simd2_amd
+    M = N = K = 2048
+    for op in ("min_plus", "max_min"):
+        A, B, C = semiring_inputs(op, M, N, K, seed=21)
+        say(f"large {op} {M}^3")
+        D = run(gpu, op, A, B, C, True)
+        rows = [0, 1, 127, 128, 1000, 2047]
+        for r in rows:
+            want = oracle_mod.srgemm_f32(op, A[r:r + 1], B, C[r:r + 1], True)
+            assert np.array_equal(bits(D[r:r + 1]), bits(want)), (op, r)
+    del torch, simd2_amd
+
+
+def test_dropin_column_major_strided(gpu, oracle_mod):
+    """Column-major call with leading dimensions larger than the matrices,
+    exactly as kernel/srgemm.cuh takes them."""
+    import torch
+    import simd2_amd
+    M, N, K = 70, 50, 90
+    lda, ldb, ldc = 80, 100, 72
+    rng = np.random.default_rng(5)
+    Acm = rng.integers(1, 100, (K, lda)).astype(np.float32)   # column k at row k
+    Bcm = rng.integers(1, 100, (N, ldb)).astype(np.float32)
+    Ccm = rng.integers(1, 100, (N, ldc)).astype(np.float32)
+    tA, tB, tC = (torch.from_numpy(x).cuda() for x in (Acm, Bcm, Ccm))
+    tD = torch.full((N, ldc), -7.0, device="cuda")
+    st = simd2_amd.cuasr_minplus_srsgemm(M, N, K, tA, lda, tB, ldb, tC, ldc, tD, True)
+    assert st == 0
+    torch.cuda.synchronize()
+    A = Acm[:, :M].T  # A(i,k)
+    B = Bcm[:, :K].T  # B(k,j)
+    C = Ccm[:, :M].T
+    want = oracle_mod.srgemm_f32("min_plus", A, B, C, True)
+    got = tD.cpu().numpy()
+    assert np.array_equal(got[:, :M].T, want)
+    assert np.all(got[:, M:] == -7.0)  # padding between columns untouched
+
+
+def test_c_equals_d(gpu, oracle_mod):
+    import torch
+    import simd2_amd
+    A, B, C = semiring_inputs("min_plus", 128, 128, 64, seed=3)
+    tA, tB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    tC = torch.from_numpy(C).cuda()
+    simd2_amd.srgemm("min_plus", tA, tB, tC, True, out=tC)
+    torch.cuda.synchronize()
+    assert np.array_equal(tC.cpu().numpy(), oracle_mod.srgemm_f32("min_plus", A, B, C, True))
+
+
+def test_or_and_int_and_packed(gpu, oracle_mod):
+    import torch
+    import simd2_amd
+    for (M, N, K, p) in [(300, 200, 500, 0.02), (64, 2048, 64, 0.1), (1, 1, 1, 1.0),
+                         (257, 95, 33, 0.3)]:
+        rng = np.random.default_rng(M + N)
+        A = (rng.random((M, K)) < p).astype(np.int32)
+        B = (rng.random((K, N)) < p).astype(np.int32)
+        C = (rng.random((M, N)) < p).astype(np.int32)
+        for epi in (False, True):
+            say(f"or_and {M}x{N}x{K} p={p} epi={epi}")
+            want = oracle_mod.srgemm_i32(A, B, C, epi)
+            assert np.array_equal(run(gpu, "or_and", A, B, C, epi), want)
+            pa, pb, pc = (simd2_amd.pack_bits(torch.from_numpy(x).cuda()) for x in (A, B, C))
+            D = simd2_amd.orand_packed(pa, pb, N, pc, epi)
+            torch.cuda.synchronize()
+            assert np.array_equal(simd2_amd.unpack_bits(D, N).cpu().numpy(), want)
+
+
+def test_k_zero_and_empty(gpu):
+    import torch
+    import simd2_amd
+    A = torch.empty((5, 0), device="cuda")
+    B = torch.empty((0, 6), device="cuda")
+    C = torch.arange(30, dtype=torch.float32, device="cuda").view(5, 6)
+    D = simd2_amd.srgemm("min_plus", A, B, C, True)
+    assert torch.equal(D, C)
+    D = simd2_amd.srgemm("max_plus", A, B, C, False)
+    assert torch.all(D == float("-inf"))
+    D = simd2_amd.srgemm("plus_mul", A, B, C, False)
+    assert torch.all(D == 0)
