@@ -63,6 +63,7 @@ EXPORTS = list(DROPIN.values()) + [
     "simd2_srgemm", "simd2_orand_packed", "simd2_status_string", "simd2_device_info",
     "comp_update", "comp_update2", "comp_update_int",
     "simd2_converge_f32", "simd2_converge_i32", "simd2_converge_orand_packed",
+    "tensor_srgemm", "simd2_tensor_srgemm", "f2h_device", "h2f_device",
 ]
 
 
@@ -110,6 +111,14 @@ def lib() -> ctypes.CDLL:
         L.simd2_converge_i32.restype = i
         L.simd2_converge_orand_packed.argtypes = [i, p, i, p, i, p, i, p, ctypes.POINTER(i)]
         L.simd2_converge_orand_packed.restype = i
+        L.tensor_srgemm.argtypes = [p, p, p, p, i, i, i, b, i]
+        L.tensor_srgemm.restype = i
+        L.simd2_tensor_srgemm.argtypes = [p, p, p, p, i, i, i, i, i, p]
+        L.simd2_tensor_srgemm.restype = i
+        L.f2h_device.argtypes = [p, p, i]
+        L.f2h_device.restype = None
+        L.h2f_device.argtypes = [p, p, i]
+        L.h2f_device.restype = None
         _lib = L
     return _lib
 
@@ -253,6 +262,37 @@ def converge(op, X, epilogue=True, check="exact", max_itr=500, flags=CONVERGE_FU
     if st not in (SUCCESS, ERROR_NOT_CONVERGED):
         raise Simd2Error(st, "converge")
     return X, itr.value, st
+
+
+def tensor_srgemm(op, A16, B16, C=None, epilogue=False, out=None, stream=None):
+    """FP16-input emulation GEMM (kernel/tensor_srgemm.cuh semantics), row-major
+    torch tensors: A16 (m x k) and B16 (k x n) float16, C and the result float32."""
+    torch = _torch()
+    m, k = A16.shape
+    n = B16.shape[1]
+    if out is None:
+        out = torch.empty((m, n), dtype=torch.float32, device=A16.device)
+    opc = OP[op] if isinstance(op, str) else int(op)
+    st = lib().simd2_tensor_srgemm(_ptr(A16), _ptr(B16), _ptr(C) if epilogue else None,
+                                   _ptr(out), m, n, k, int(bool(epilogue)), opc, _stream(stream))
+    _check(st, "tensor_srgemm")
+    return out
+
+
+def f2h(x):
+    """f2h_device (kernel/precision.cuh): float32 -> float16, synchronous."""
+    torch = _torch()
+    out = torch.empty(x.shape, dtype=torch.float16, device=x.device)
+    lib().f2h_device(_ptr(out), _ptr(x), x.numel())
+    return out
+
+
+def h2f(x):
+    """h2f_device (kernel/precision.cuh): float16 -> float32, synchronous."""
+    torch = _torch()
+    out = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+    lib().h2f_device(_ptr(out), _ptr(x), x.numel())
+    return out
 
 
 def device_info():

@@ -9,6 +9,7 @@
 
 #include "../../include/simd2/converge.h"
 #include "../../include/simd2/srgemm.h"
+#include "../../include/simd2/tensor_emu.h"
 #include "internal.hpp"
 
 using namespace simd2;
@@ -174,6 +175,34 @@ int simd2_device_info(int *cu_count, int *clock_khz, char *name, int name_len) {
     name[name_len - 1] = 0;
   }
   return SIMD2_SUCCESS;
+}
+
+// ---- FP16 emulation --------------------------------------------------------
+
+int simd2_tensor_srgemm(const simd2_half *a, const simd2_half *b, const float *c, float *d,
+                        int m, int n, int k, int do_epilogue, int opcode, hipStream_t stream) {
+  if (m < 0 || n < 0 || k < 0 || m % 16 || n % 16 || k % 16) return -1;
+  if (opcode < 0 || opcode > 8) return SIMD2_ERROR_NOT_SUPPORTED;
+  if (m == 0 || n == 0) return SIMD2_SUCCESS;
+  if (!d || (k > 0 && (!a || !b)) || (do_epilogue && !c)) return SIMD2_ERROR_NULL_POINTER;
+  return hip_status(launch_tensor_emu(opcode, a, b, c, d, m, n, k, do_epilogue, stream));
+}
+
+int tensor_srgemm(simd2_half *a, simd2_half *b, float *c, float *d, int m, int n, int k,
+                  bool do_epilogue, int opcode) {
+  return simd2_tensor_srgemm(a, b, c, d, m, n, k, do_epilogue, opcode, nullptr);
+}
+
+void f2h_device(simd2_half *fp16_d, float *fp32_d, int num_elements) {
+  if (!fp16_d || !fp32_d || num_elements <= 0) return;
+  if (launch_f2h(fp16_d, fp32_d, num_elements, nullptr) == hipSuccess)
+    (void)hipDeviceSynchronize();
+}
+
+void h2f_device(float *fp32_d, simd2_half *fp16_d, int num_elements) {
+  if (!fp16_d || !fp32_d || num_elements <= 0) return;
+  if (launch_h2f(fp32_d, fp16_d, num_elements, nullptr) == hipSuccess)
+    (void)hipDeviceSynchronize();
 }
 
 // ---- convergence --------------------------------------------------------

@@ -54,6 +54,12 @@ hipError_t launch_orand_packed(int M, int N, int K, const uint32_t *A,
                                int64_t ldd, const uint32_t *P, int64_t ldp,
                                int *flag, hipStream_t s);
 
+// FP16-input emulation kernel and conversions (tensor_emu.hip).
+hipError_t launch_tensor_emu(int op, const void *a, const void *b, const float *c, float *d,
+                             int m, int n, int k, int do_epilogue, hipStream_t s);
+hipError_t launch_f2h(void *out, const float *in, int64_t n, hipStream_t s);
+hipError_t launch_h2f(float *out, const void *in, int64_t n, hipStream_t s);
+
 // Element-wise helpers (converge.hip).
 hipError_t launch_comp_update_f32(float *prev, float *cur, int *flag,
                                   int64_t n, int mode, hipStream_t s);

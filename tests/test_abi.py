@@ -11,7 +11,8 @@ from conftest import ROOT
 
 import simd2_amd
 
-HEADERS = [os.path.join(ROOT, "include", "simd2", h) for h in ("srgemm.h", "converge.h")]
+HEADERS = [os.path.join(ROOT, "include", "simd2", h)
+           for h in ("srgemm.h", "converge.h", "tensor_emu.h")]
 
 
 def declared_functions():
