@@ -127,6 +127,30 @@ int simd2_srgemm(int op, int M, int N, int K, const void *A, int lda,
                  const void *B, int ldb, const void *C, int ldc, void *D,
                  int ldd, int do_epilogue, hipStream_t stream);
 
+/* simd2_srgemm with an explicit tile configuration for the VALU semirings
+ * (no counterpart in the reference, whose tile shape is fixed at compile
+ * time, kernel/srgemm.cuh:574-583): -1 = automatic, 0 = 128x128 tiles (8x8
+ * per thread), 1 = 64x64 (4x4), 2 = 64x128 (4x8 per thread; 64 along M and
+ * 128 along N of the column-major call).  Used by the autotuner and benchmarks; results are
+ * identical for every configuration. */
+int simd2_srgemm_ex(int op, int M, int N, int K, const void *A, int lda, const void *B,
+                    int ldb, const void *C, int ldc, void *D, int ldd, int do_epilogue,
+                    int tile_config, hipStream_t stream);
+
+/* Number of tile configurations, and the one chosen automatically for op on
+ * a column-major M x N x K problem. */
+int simd2_tile_config_count(void);
+int simd2_pick_tile_config(int op, int M, int N, int K);
+
+/* Autotuner: times every tile configuration on this exact call (each run
+ * writes the same D, so D holds the correct result afterwards), records the
+ * fastest for (op, M, N, K) -- later automatic calls of that shape use it --
+ * and returns it in *best; ms[cfg] receives each configuration's mean time
+ * when ms is not null.  Synchronises `stream`.  Not for or_and / plus_mul. */
+int simd2_autotune(int op, int M, int N, int K, const void *A, int lda, const void *B, int ldb,
+                   const void *C, int ldc, void *D, int ldd, int do_epilogue,
+                   hipStream_t stream, int *best, float *ms);
+
 /* Bit-packed or_and (no counterpart in the reference, whose or_and stores one
  * boolean per int32).  Row-major bit matrices: bit (k & 31) of word
  * A[i*lda_words + k/32] is A(i,k); likewise B (K x N bits) and C, D (M x N

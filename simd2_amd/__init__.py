@@ -64,6 +64,7 @@ EXPORTS = list(DROPIN.values()) + [
     "comp_update", "comp_update2", "comp_update_int",
     "simd2_converge_f32", "simd2_converge_i32", "simd2_converge_orand_packed",
     "tensor_srgemm", "simd2_tensor_srgemm", "f2h_device", "h2f_device",
+    "simd2_srgemm_ex", "simd2_tile_config_count", "simd2_pick_tile_config", "simd2_autotune",
 ]
 
 
@@ -95,6 +96,15 @@ def lib() -> ctypes.CDLL:
             fn.restype = i
         L.simd2_srgemm.argtypes = [i, i, i, i, p, i, p, i, p, i, p, i, i, p]
         L.simd2_srgemm.restype = i
+        L.simd2_srgemm_ex.argtypes = [i, i, i, i, p, i, p, i, p, i, p, i, i, i, p]
+        L.simd2_srgemm_ex.restype = i
+        L.simd2_tile_config_count.argtypes = []
+        L.simd2_tile_config_count.restype = i
+        L.simd2_pick_tile_config.argtypes = [i, i, i, i]
+        L.simd2_pick_tile_config.restype = i
+        L.simd2_autotune.argtypes = [i, i, i, i, p, i, p, i, p, i, p, i, i, p,
+                                     ctypes.POINTER(i), ctypes.POINTER(ctypes.c_float)]
+        L.simd2_autotune.restype = i
         L.simd2_orand_packed.argtypes = [i, i, i, p, i, p, i, p, i, p, i, i, p]
         L.simd2_orand_packed.restype = i
         L.simd2_status_string.argtypes = [i]
@@ -184,10 +194,11 @@ def simd2_srgemm(op, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, do_epilogue, strea
                               ldd, int(bool(do_epilogue)), _stream(stream))
 
 
-def srgemm(op, A, B, C=None, epilogue=False, out=None, stream=None):
+def srgemm(op, A, B, C=None, epilogue=False, out=None, stream=None, tile_config=None):
     """Row-major ``D = [C (+)] A (x) B`` for torch device tensors (float32, or
     int32 for or_and), through the column-major drop-in the way the
-    reference's tests call it.  Returns D."""
+    reference's tests call it (``tile_config`` selects the VALU tile shape
+    through simd2_srgemm_ex).  Returns D."""
     torch = _torch()
     opname = op if isinstance(op, str) else OPS[op]
     M, K = A.shape
@@ -197,10 +208,35 @@ def srgemm(op, A, B, C=None, epilogue=False, out=None, stream=None):
     dt = torch.int32 if opname == "or_and" else torch.float32
     if out is None:
         out = torch.empty((M, N), dtype=dt, device=A.device)
-    st = cuasr_srsgemm(opname, N, M, K, B, max(N, 1), A, max(K, 1),
-                       C if epilogue else None, max(N, 1), out, bool(epilogue), stream)
+    if tile_config is None:
+        st = cuasr_srsgemm(opname, N, M, K, B, max(N, 1), A, max(K, 1),
+                           C if epilogue else None, max(N, 1), out, bool(epilogue), stream)
+    else:
+        st = lib().simd2_srgemm_ex(OP[opname], N, M, K, _ptr(B), max(N, 1), _ptr(A), max(K, 1),
+                                   _ptr(C) if epilogue else None, max(N, 1), _ptr(out),
+                                   max(N, 1), int(bool(epilogue)), int(tile_config),
+                                   _stream(stream))
     _check(st, DROPIN[opname])
     return out
+
+
+def autotune(op, A, B, C=None, epilogue=False, out=None, stream=None):
+    """Autotune the VALU tile configuration for the row-major problem
+    D = [C (+)] A (x) B (simd2_autotune).  Returns (D, best_config, ms_per_config)."""
+    torch = _torch()
+    opname = op if isinstance(op, str) else OPS[op]
+    M, K = A.shape
+    N = B.shape[1]
+    if out is None:
+        out = torch.empty((M, N), dtype=torch.float32, device=A.device)
+    best = ctypes.c_int(-1)
+    ms = (ctypes.c_float * 8)()
+    st = lib().simd2_autotune(OP[opname], N, M, K, _ptr(B), max(N, 1), _ptr(A), max(K, 1),
+                              _ptr(C) if epilogue else None, max(N, 1), _ptr(out), max(N, 1),
+                              int(bool(epilogue)), _stream(stream), ctypes.byref(best), ms)
+    _check(st, "simd2_autotune")
+    n = lib().simd2_tile_config_count()
+    return out, best.value, [ms[i] for i in range(n)]
 
 
 def orand_packed(A, B, N, C=None, epilogue=False, out=None, stream=None):

@@ -56,7 +56,7 @@ int check_cm(int M, int N, int K, const void *A, int lda, const void *B, int ldb
 // X = B (rows N, k contiguous, ldb) and Y = A (K rows, M contiguous, lda).
 int srgemm_f32_cm(int op, int M, int N, int K, const float *A, int lda, const float *B,
                   int ldb, const float *C, int ldc, float *D, int ldd, bool epi,
-                  hipStream_t s) {
+                  hipStream_t s, int cfg = CFG_AUTO) {
   if (op < 0 || op > 8 || op == SIMD2_OR_AND) return SIMD2_ERROR_NOT_SUPPORTED;
   const int st = check_cm(M, N, K, A, lda, B, ldb, C, ldc, D, ldd, epi, sizeof(float));
   if (st == OK_NOTHING_TO_DO) return SIMD2_SUCCESS;
@@ -70,7 +70,7 @@ int srgemm_f32_cm(int op, int M, int N, int K, const float *A, int lda, const fl
       ep.C = C;
       ep.ldc = ldc;
     }
-    e = launch_valu_f32(op, N, M, K, B, ldb, A, lda, D, ldd, ep, s);
+    e = launch_valu_f32(op, N, M, K, B, ldb, A, lda, D, ldd, ep, s, cfg);
   }
   return hip_status(e);
 }
@@ -138,6 +138,63 @@ int simd2_srgemm(int op, int M, int N, int K, const void *A, int lda, const void
   return srgemm_f32_cm(op, M, N, K, static_cast<const float *>(A), lda,
                        static_cast<const float *>(B), ldb, static_cast<const float *>(C),
                        ldc, static_cast<float *>(D), ldd, do_epilogue != 0, stream);
+}
+
+int simd2_srgemm_ex(int op, int M, int N, int K, const void *A, int lda, const void *B,
+                    int ldb, const void *C, int ldc, void *D, int ldd, int do_epilogue,
+                    int tile_config, hipStream_t stream) {
+  if (tile_config < -1 || tile_config >= CFG_COUNT) return SIMD2_ERROR_NOT_SUPPORTED;
+  if (op == SIMD2_OR_AND || op == SIMD2_PLUS_MUL)
+    return simd2_srgemm(op, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, do_epilogue, stream);
+  return srgemm_f32_cm(op, M, N, K, static_cast<const float *>(A), lda,
+                       static_cast<const float *>(B), ldb, static_cast<const float *>(C),
+                       ldc, static_cast<float *>(D), ldd, do_epilogue != 0, stream,
+                       tile_config);
+}
+
+int simd2_tile_config_count(void) { return CFG_COUNT; }
+
+int simd2_pick_tile_config(int op, int M, int N, int K) { return valu_pick_cfg(op, N, M, K); }
+
+int simd2_autotune(int op, int M, int N, int K, const void *A, int lda, const void *B, int ldb,
+                   const void *C, int ldc, void *D, int ldd, int do_epilogue, hipStream_t stream,
+                   int *best, float *ms) {
+  if (best) *best = -1;
+  if (op < 0 || op > 8 || op == SIMD2_OR_AND || op == SIMD2_PLUS_MUL) return SIMD2_ERROR_NOT_SUPPORTED;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return SIMD2_ERROR_INTERNAL;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return SIMD2_ERROR_INTERNAL;
+  }
+  int st = SIMD2_SUCCESS, arg = -1;
+  float best_ms = 1e30f;
+  for (int cfg = 0; cfg < CFG_COUNT && st == SIMD2_SUCCESS; ++cfg) {
+    st = simd2_srgemm_ex(op, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, do_epilogue, cfg, stream);
+    if (st != SIMD2_SUCCESS) break;
+    float t = 0.f;
+    if (hipEventRecord(e0, stream) != hipSuccess) { st = SIMD2_ERROR_INTERNAL; break; }
+    for (int r = 0; r < 3 && st == SIMD2_SUCCESS; ++r)
+      st = simd2_srgemm_ex(op, M, N, K, A, lda, B, ldb, C, ldc, D, ldd, do_epilogue, cfg, stream);
+    if (st != SIMD2_SUCCESS) break;
+    if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+      st = SIMD2_ERROR_INTERNAL;
+      break;
+    }
+    t /= 3.f;
+    if (ms) ms[cfg] = t;
+    if (t < best_ms) {
+      best_ms = t;
+      arg = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != SIMD2_SUCCESS) return st;
+  valu_record_cfg(op, N, M, K, arg);
+  if (best) *best = arg;
+  return SIMD2_SUCCESS;
 }
 
 int simd2_orand_packed(int M, int N, int K, const uint32_t *A, int lda_words,

@@ -25,10 +25,11 @@ struct Epilogue {
 enum CheckMode { CHECK_EXACT = 0, CHECK_THRESHOLD = 1, CHECK_APLP = 2 };
 
 // Tile configurations of the VALU kernel.
-enum ValuCfg { CFG_AUTO = -1, CFG_L128 = 0, CFG_S64 = 1, CFG_COUNT = 2 };
+enum ValuCfg { CFG_AUTO = -1, CFG_L128 = 0, CFG_S64 = 1, CFG_M128x64 = 2, CFG_COUNT = 3 };
 struct ValuCfgInfo { int bm, bn, bk, threads; };
 ValuCfgInfo valu_cfg_info(int cfg);
-int valu_pick_cfg(int R, int Cn, int K);
+int valu_pick_cfg(int op, int R, int Cn, int K);
+void valu_record_cfg(int op, int R, int Cn, int K, int cfg);
 
 hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,
                            int64_t ldx, const float *Y, int64_t ldy, float *O,

@@ -22,6 +22,8 @@
 // visited in groups of GROUP tile-rows so neighbours share panels in L2.
 #include <float.h>
 
+#include <mutex>
+
 #include "internal.hpp"
 #include "semiring_ops.hpp"
 
@@ -172,10 +174,12 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
 
   for (int kt = 0; kt < nkt; ++kt) {
     const int buf = kt & 1;
-    const bool more = kt + 1 < nkt;
-    if (more)
-      load_slab<Op, BM, BN, BK, NT, CHECKED>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0,
-                                             (kt + 1) * BK, tid);
+    // The prefetch is unconditional (the last slab is simply loaded again and
+    // written to the idle buffer, which nothing reads afterwards): under a
+    // branch the compiler keeps the prefetch in scratch and waits for it.
+    const int knext = (kt + 1 < nkt ? kt + 1 : kt) * BK;
+    load_slab<Op, BM, BN, BK, NT, CHECKED>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, knext,
+                                           tid);
 #pragma unroll
     for (int k = 0; k < BK; k += 2) {
       // Keep each k-pair's LDS reads next to its arithmetic: without this
@@ -207,7 +211,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
         }
       }
     }
-    if (more) store_slab<BM, BN, BK, NT, XLD>(Xs[buf ^ 1], Ys[buf ^ 1], xv, yv, tid);
+    store_slab<BM, BN, BK, NT, XLD>(Xs[buf ^ 1], Ys[buf ^ 1], xv, yv, tid);
     __syncthreads();
   }
 
@@ -272,6 +276,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
 // WAVES: waves per SIMD the register allocation must leave room for
 // (3 -> at most 168 VGPRs for the 8x8 register tile).
 struct CfgL { static constexpr int BM = 128, BN = 128, TM = 8, TN = 8, BK = 16, WAVES = 3; };
+struct CfgM { static constexpr int BM = 128, BN = 64, TM = 8, TN = 4, BK = 16, WAVES = 4; };
 struct CfgS { static constexpr int BM = 64, BN = 64, TM = 4, TN = 4, BK = 16, WAVES = 4; };
 
 template <class Op, class Cfg>
@@ -302,28 +307,78 @@ hipError_t launch_op(int R, int Cn, int K, const float *X, int64_t ldx,
                      const float *Y, int64_t ldy, float *O, int64_t ldo,
                      const Epilogue &e, hipStream_t s, int cfg) {
   if (cfg == CFG_S64) return launch_cfg<Op, CfgS>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+  if (cfg == CFG_M128x64) return launch_cfg<Op, CfgM>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
   return launch_cfg<Op, CfgL>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+}
+
+int device_cu_count() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+    else
+      cus = 256;
+  }
+  return cus;
 }
 
 }  // namespace
 
 ValuCfgInfo valu_cfg_info(int cfg) {
   if (cfg == CFG_S64) return {CfgS::BM, CfgS::BN, CfgS::BK, 256};
+  if (cfg == CFG_M128x64) return {CfgM::BM, CfgM::BN, CfgM::BK, 256};
   return {CfgL::BM, CfgL::BN, CfgL::BK, 256};
 }
 
-// Large tiles unless they leave most of the 256 CUs idle.
-int valu_pick_cfg(int R, int Cn, int K) {
-  (void)K;
-  const long tiles_l = (long)((R + 127) / 128) * ((Cn + 127) / 128);
-  return tiles_l >= 512 ? CFG_L128 : CFG_S64;
+// Tile shape choice.  Rule measured on MI355X for square problems
+// (n = 1024 .. 8192, all ops): the VALU semirings need many resident waves to
+// dual-issue, so up to ~3072^2 outputs the 64x64 tiles (7-8 waves per SIMD)
+// win; at 4096^2 the 128x64 tiles balance 2048 tiles over 1024 slots; from
+// ~6144^2 up the 128x128 tiles (fewest LDS reads per op) win, except for
+// minus_square whose sub+fma pair keeps preferring occupancy.  Shapes the
+// autotuner has timed (simd2_autotune) use the recorded winner instead.
+namespace {
+struct TunedEntry { int op, R, Cn, K, cfg; };
+constexpr int TUNED_MAX = 64;
+TunedEntry g_tuned[TUNED_MAX];
+int g_tuned_n = 0;
+std::mutex g_tuned_mu;
+}  // namespace
+
+void valu_record_cfg(int op, int R, int Cn, int K, int cfg) {
+  std::lock_guard<std::mutex> lock(g_tuned_mu);
+  for (int i = 0; i < g_tuned_n; ++i)
+    if (g_tuned[i].op == op && g_tuned[i].R == R && g_tuned[i].Cn == Cn && g_tuned[i].K == K) {
+      g_tuned[i].cfg = cfg;
+      return;
+    }
+  const int slot = g_tuned_n < TUNED_MAX ? g_tuned_n++ : (op + R + Cn + K) % TUNED_MAX;
+  g_tuned[slot] = {op, R, Cn, K, cfg};
+}
+
+int valu_pick_cfg(int op, int R, int Cn, int K) {
+  {
+    std::lock_guard<std::mutex> lock(g_tuned_mu);
+    for (int i = 0; i < g_tuned_n; ++i)
+      if (g_tuned[i].op == op && g_tuned[i].R == R && g_tuned[i].Cn == Cn && g_tuned[i].K == K)
+        return g_tuned[i].cfg;
+  }
+  const double area = (double)R * Cn;
+  const double scale = 256.0 / device_cu_count();   // rule is for 256 CUs
+  const double a = area * scale;
+  if (a <= 3072.0 * 3072.0 * 1.01) return CFG_S64;
+  if (op == 8) return CFG_S64;
+  if (a <= 4096.0 * 4096.0 * 1.01) return CFG_M128x64;
+  return CFG_L128;
 }
 
 hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,
                            int64_t ldx, const float *Y, int64_t ldy, float *O,
                            int64_t ldo, const Epilogue &e, hipStream_t s,
                            int cfg) {
-  if (cfg == CFG_AUTO) cfg = valu_pick_cfg(R, Cn, K);
+  if (cfg == CFG_AUTO) cfg = valu_pick_cfg(op, R, Cn, K);
   switch (op) {
   case 0: return launch_op<MinPlus>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s, cfg);
   case 1: return launch_op<PlusMul>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s, cfg);

@@ -200,3 +200,35 @@ def test_k_zero_and_empty(gpu):
     assert torch.all(D == float("-inf"))
     D = simd2_amd.srgemm("plus_mul", A, B, C, False)
     assert torch.all(D == 0)
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 384, 128), (200, 130, 70)])
+def test_every_tile_config_identical(gpu, oracle_mod, M, N, K):
+    """Each VALU tile configuration (128x128, 64x64, 128x64 tiles) returns the
+    oracle's bits, on and off the tile grid."""
+    import simd2_amd
+    n_cfg = simd2_amd.lib().simd2_tile_config_count()
+    assert n_cfg == 3
+    for op in ("min_plus", "max_min", "min_mul"):
+        A, B, C = semiring_inputs(op, M, N, K, seed=M * 7 + K)
+        want = oracle_mod.srgemm_f32(op, A, B, C, True, threads=8)
+        tA, tB, tC = (gpu.from_numpy(x).cuda() for x in (A, B, C))
+        for cfg in range(n_cfg):
+            say(f"cfg {cfg} {op} {M}x{N}x{K}")
+            D = simd2_amd.srgemm(op, tA, tB, tC, True, tile_config=cfg)
+            gpu.cuda.synchronize()
+            assert np.array_equal(bits(D.cpu().numpy()), bits(want)), (op, cfg)
+
+
+def test_autotune_records_winner(gpu, oracle_mod):
+    import simd2_amd
+    M = N = K = 512
+    A, B, C = semiring_inputs("min_plus", M, N, K, seed=17)
+    tA, tB, tC = (gpu.from_numpy(x).cuda() for x in (A, B, C))
+    D, best, ms = simd2_amd.autotune("min_plus", tA, tB, tC, True)
+    gpu.cuda.synchronize()
+    assert best in (0, 1, 2) and all(t > 0 for t in ms)
+    assert ms[best] == min(ms)
+    want = oracle_mod.srgemm_f32("min_plus", A, B, C, True, threads=8)
+    assert np.array_equal(bits(D.cpu().numpy()), bits(want))
+    assert simd2_amd.lib().simd2_pick_tile_config(0, N, M, K) == best
