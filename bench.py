@@ -189,6 +189,11 @@ def main():
     calls = {}
     for i, op in enumerate(OPS):
         calls[op] = make_call(torch, simd2_amd, op, n, seed=1000 + i)
+    # Traffic calibration for the rocprofv3 FETCH_SIZE / WRITE_SIZE passes: one
+    # plain streaming kernel (torch.neg) of exactly 48 MiB in and 48 MiB out.
+    cal = torch.ones(12 * 1024 * 1024, device="cuda")
+    torch.neg(cal, out=torch.empty_like(cal))
+    del cal
     torch.cuda.synchronize()
 
     def step():
