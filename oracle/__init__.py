@@ -68,6 +68,10 @@ def _load():
         lib.oracle_comp_update_exact_f32.argtypes = [_f32p, _f32p, ll]
         lib.oracle_converge_f32.argtypes = [i, i, _f32p, i, i, i, i]
         lib.oracle_converge_i32.argtypes = [i, _i32p, i]
+        lib.oracle_knn_insertion_sort.argtypes = [_f32p, _i32p, i, i, i]
+        lib.oracle_knn_insertion_sort.restype = None
+        lib.oracle_knn_sqrt.argtypes = [_f32p, i, i]
+        lib.oracle_knn_sqrt.restype = None
         _lib = lib
     return _lib
 
@@ -186,6 +190,21 @@ def converge_i32(X, max_itr=500):
     X = _c(X, np.int32).copy()
     itr = _load().oracle_converge_i32(X.shape[0], X, max_itr)
     return X, itr
+
+
+def knn_insertion_sort(dist_rq, k, do_sqrt=True):
+    """modified_insertion_sort_linear + compute_sqrt_linear
+    (apps/pld/cuASR/knncuda.cu:170-237) on a distance matrix in the
+    reference layout: dist_rq has shape (R, Q) (row r = distances of all
+    queries to reference r).  Returns (dist k x Q, index k x Q)."""
+    D = _c(dist_rq, np.float32).copy()
+    R, Q = D.shape
+    idx = np.zeros((max(R, k), Q), np.int32)
+    _load().oracle_knn_insertion_sort(D, idx, Q, R, k)
+    out = np.ascontiguousarray(D[:k])
+    if do_sqrt:
+        _load().oracle_knn_sqrt(out, Q, k)
+    return out, np.ascontiguousarray(idx[:k])
 
 
 # ---- the reference's own host_srgemm (only where the reference tree exists) ----

@@ -80,6 +80,10 @@ int oracle_converge_f32(int op, int n, float *X, int do_epilogue,
                         int check_mode, int max_itr, int nthreads);
 int oracle_converge_i32(int n, int32_t *X, int max_itr);
 
+/* PLD post-processing (apps/pld/cuASR/knncuda.cu:170-237), see knn.c. */
+void oracle_knn_insertion_sort(float *dist, int32_t *index, int width, int height, int k);
+void oracle_knn_sqrt(float *dist, int width, int k);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,7 @@ ERROR_NULL_POINTER = 11
 ERROR_ALIASING = 12
 ERROR_NO_DEVICE = 13
 ERROR_NOT_CONVERGED = 14
+ERROR_WORKSPACE = 15
 
 CHECK = {"exact": 0, "threshold": 1, "aplp": 2}
 CONVERGE_REFERENCE = 0
@@ -65,6 +66,7 @@ EXPORTS = list(DROPIN.values()) + [
     "simd2_converge_f32", "simd2_converge_i32", "simd2_converge_orand_packed",
     "tensor_srgemm", "simd2_tensor_srgemm", "f2h_device", "h2f_device",
     "simd2_srgemm_ex", "simd2_tile_config_count", "simd2_pick_tile_config", "simd2_autotune",
+    "simd2_knn_select", "knn_mxu", "simd2_knn_bf16_workspace_size", "simd2_knn_bf16",
 ]
 
 

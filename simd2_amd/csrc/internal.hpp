@@ -61,6 +61,14 @@ hipError_t launch_tensor_emu(int op, const void *a, const void *b, const float *
 hipError_t launch_f2h(void *out, const float *in, int64_t n, hipStream_t s);
 hipError_t launch_h2f(float *out, const void *in, int64_t n, hipStream_t s);
 
+// k-nearest neighbours (knn.hip).
+hipError_t launch_knn_select(const float *dist, int Q, int R, int k, float *out_d, int *out_i,
+                             int do_sqrt, hipStream_t s);
+size_t knn_bf16_workspace(int Q, int R, int d, int *slices_out);
+hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, int64_t ldr, int Q,
+                           int R, int d, int k, float *out_d, int *out_i, int do_sqrt,
+                           void *workspace, int *stats, hipStream_t s);
+
 // Element-wise helpers (converge.hip).
 hipError_t launch_comp_update_f32(float *prev, float *cur, int *flag,
                                   int64_t n, int mode, hipStream_t s);

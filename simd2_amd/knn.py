@@ -1,0 +1,95 @@
+"""k-nearest neighbours over the C ABI (include/simd2/knn.h): the
+reference's PLD post-processing (knn_select, knn_mxu) and the BF16-MFMA
+precision form (knn_bf16)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _check, _ptr, _stream, _torch, lib
+
+SQRT = 1
+
+
+def _bind():
+    L = lib()
+    if getattr(L, "_knn_bound", False):
+        return L
+    i, p = ctypes.c_int, ctypes.c_void_p
+    L.simd2_knn_select.argtypes = [p, i, i, i, p, p, i, p]
+    L.simd2_knn_select.restype = i
+    L.simd2_knn_bf16_workspace_size.argtypes = [i, i, i]
+    L.simd2_knn_bf16_workspace_size.restype = ctypes.c_size_t
+    L.simd2_knn_bf16.argtypes = [p, i, p, i, i, i, i, i, p, p, i, p, ctypes.c_size_t, p, p]
+    L.simd2_knn_bf16.restype = i
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    L.knn_mxu.argtypes = [f32p, i, f32p, i, i, i, f32p, i32p, ctypes.POINTER(ctypes.c_double)]
+    L.knn_mxu.restype = ctypes.c_bool
+    L._knn_bound = True
+    return L
+
+
+def knn_select(dist_rq, k, sqrt=True, stream=None):
+    """Top-k per query of a distance matrix in the reference layout
+    (dist_rq: R x Q device tensor, row r = distances of all queries to
+    reference r).  Returns (dist k x Q, index k x Q)."""
+    torch = _torch()
+    R, Q = dist_rq.shape
+    out_d = torch.empty((k, Q), dtype=torch.float32, device=dist_rq.device)
+    out_i = torch.empty((k, Q), dtype=torch.int32, device=dist_rq.device)
+    st = _bind().simd2_knn_select(_ptr(dist_rq), Q, R, k, _ptr(out_d), _ptr(out_i),
+                                  SQRT if sqrt else 0, _stream(stream))
+    _check(st, "simd2_knn_select")
+    return out_d, out_i
+
+
+def knn_mxu(ref, query, k):
+    """The reference's knn_mxu on host arrays: ref (R x dim), query
+    (dim x Q, dimension-major).  Returns (dist k x Q, index k x Q, seconds)."""
+    ref = np.ascontiguousarray(ref, np.float32)
+    query = np.ascontiguousarray(query, np.float32)
+    R, dim = ref.shape
+    Q = query.shape[1]
+    out_d = np.zeros((k, Q), np.float32)
+    out_i = np.zeros((k, Q), np.int32)
+    t = ctypes.c_double(0.0)
+    ok = _bind().knn_mxu(ref, R, query, Q, dim, k, out_d, out_i, ctypes.byref(t))
+    if not ok:
+        raise RuntimeError("knn_mxu failed")
+    return out_d, out_i, t.value
+
+
+class KnnBF16:
+    """Reusable precision-form kNN: workspace allocated once per shape."""
+
+    def __init__(self, Q, R, dim, device="cuda"):
+        torch = _torch()
+        self.Q, self.R, self.dim = Q, R, dim
+        nbytes = _bind().simd2_knn_bf16_workspace_size(Q, R, dim)
+        self.ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+        self.stats = torch.zeros(2, dtype=torch.int32, device=device)
+
+    def __call__(self, query, ref, k, sqrt=False, out=None, stream=None):
+        torch = _torch()
+        Q, dim = query.shape
+        R = ref.shape[0]
+        assert (Q, R, dim) == (self.Q, self.R, self.dim)
+        if out is None:
+            out = (torch.empty((Q, k), dtype=torch.float32, device=query.device),
+                   torch.empty((Q, k), dtype=torch.int32, device=query.device))
+        st = _bind().simd2_knn_bf16(_ptr(query), query.stride(0), _ptr(ref), ref.stride(0), Q, R,
+                                    dim, k, _ptr(out[0]), _ptr(out[1]), SQRT if sqrt else 0,
+                                    _ptr(self.ws), self.ws.numel(), _ptr(self.stats),
+                                    _stream(stream))
+        _check(st, "simd2_knn_bf16")
+        return out
+
+
+def knn_bf16(query, ref, k, sqrt=False, stream=None):
+    """One-shot precision-form kNN.  Returns (dist Q x k, index Q x k, stats)
+    where stats = [queries that used FP64, candidates recomputed in FP64]."""
+    runner = KnnBF16(query.shape[0], ref.shape[0], query.shape[1], device=query.device)
+    d, i = runner(query, ref, k, sqrt=sqrt, stream=stream)
+    return d, i, runner.stats

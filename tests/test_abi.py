@@ -12,7 +12,7 @@ from conftest import ROOT
 import simd2_amd
 
 HEADERS = [os.path.join(ROOT, "include", "simd2", h)
-           for h in ("srgemm.h", "converge.h", "tensor_emu.h")]
+           for h in ("srgemm.h", "converge.h", "tensor_emu.h", "knn.h")]
 
 
 def declared_functions():
@@ -20,7 +20,7 @@ def declared_functions():
     for h in HEADERS:
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^\s*(?:const\s+)?(?:int|bool|char|void)\s*\*?\s*(\w+)\s*\(",
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:int|bool|char|void|size_t)\s*\*?\s*(\w+)\s*\(",
                              text, flags=re.M):
             names.add(m.group(1))
     return names
@@ -105,3 +105,20 @@ def test_converge_argument_errors(lib):
         simd2_amd.ERROR_NULL_POINTER
     assert f(0, 0, None, 0, None, 0, None, 1, 0, 10, 1, None, ctypes.byref(itr)) == \
         simd2_amd.SUCCESS
+
+
+def test_knn_argument_errors(lib):
+    import simd2_amd.knn as K
+    L = K._bind()
+    assert L.simd2_knn_select(P1, 100, 50, 0, P2, P3, 0, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert L.simd2_knn_select(P1, 100, 50, 33, P2, P3, 0, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert L.simd2_knn_select(P1, 100, 10, 16, P2, P3, 0, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert L.simd2_knn_select(None, 100, 50, 16, P2, P3, 0, None) == simd2_amd.ERROR_NULL_POINTER
+    ws = L.simd2_knn_bf16_workspace_size(1000, 2000, 128)
+    assert ws > 1000 * 128 * 2 + 2000 * 128 * 2
+    args = (P1, 128, P2, 128, 1000, 2000, 128, 16, P3, P4, 0)
+    assert L.simd2_knn_bf16(*args, P4 + (1 << 20), ws - 1, None, None) == simd2_amd.ERROR_WORKSPACE
+    assert L.simd2_knn_bf16(P1, 128, P2, 128, 1000, 2000, 200, 16, P3, P4, 0, P4, ws, None,
+                            None) == simd2_amd.ERROR_INVALID_PROBLEM   # dim > 160
+    assert L.simd2_knn_bf16(P1, 128, P2, 128, 1000, 2000, 128, 32, P3, P4, 0, P4, ws, None,
+                            None) == simd2_amd.ERROR_INVALID_PROBLEM   # k > 31

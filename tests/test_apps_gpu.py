@@ -31,7 +31,8 @@ def run_app(args, stdin=None, tmp_path=None, dtype=np.float32, v=None):
 
 
 def edge_text(v, edges):
-    return f"{v} {len(edges)}\n" + "".join(f"{a} {b} {w:g}\n" for a, b, w in edges)
+    # 9 significant digits round-trip every float32 weight exactly
+    return f"{v} {len(edges)}\n" + "".join(f"{a} {b} {w:.9g}\n" for a, b, w in edges)
 
 
 def random_edges(v, p, seed, wlo=1, whi=100, dag=False):
@@ -85,7 +86,7 @@ def test_mst_pbbs_file(gpu, oracle_mod, tmp_path):
     with open(path, "w") as f:
         f.write("WeightedEdgeArray\n")
         for a, b, w in edges:
-            f.write(f"{a} {b} {w:g}\n")
+            f.write(f"{a} {b} {w:.9g}\n")
     vmax = max(max(a, b) for a, b, _ in edges) + 1
     X0 = np.zeros((vmax, vmax), np.float32)
     for a, b, w in edges:

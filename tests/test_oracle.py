@@ -208,3 +208,55 @@ def test_converge_cap(oracle_mod):
     X0 = np.array([[0, 1], [1, 0]], np.float32)
     _, itr = oracle_mod.converge_f32("max_plus", X0, check="exact", max_itr=7)
     assert itr == 7
+
+
+# ---- FP16 emulation restatement (kernel/tensor_srgemm.cuh) ---------------------
+
+def test_fp16_emulation_exact_inputs_equal_fp32_oracle(oracle_mod):
+    """Small integers make every half operation exact, so the emulation must
+    equal host_srgemm (max-based results are >= 0 here, where the 0
+    initial value is harmless)."""
+    from oracle import fp16_emulation as emu
+    rng = np.random.default_rng(8)
+    A = rng.integers(0, 16, (32, 48)).astype(np.float32)
+    B = rng.integers(0, 16, (48, 16)).astype(np.float32)
+    C = rng.integers(0, 16, (32, 16)).astype(np.float32)
+    for op in ["min_plus", "plus_mul", "max_plus", "max_min", "max_mul", "min_mul", "min_max",
+               "minus_square"]:
+        for epi in (False, True):
+            want = oracle_mod.srgemm_f32(op, A, B, C, epi)
+            got = emu.tensor_srgemm(A.astype(np.float16), B.astype(np.float16), C, op, epi)
+            assert np.array_equal(got, want), (op, epi)
+
+
+def test_fp16_emulation_quirks():
+    from oracle import fp16_emulation as emu
+    # max-based accumulators start at __float2half(FLT_MIN) == 0: all-negative
+    # products are clamped to 0.
+    A = np.full((16, 16), -2.0, np.float16)
+    B = np.full((16, 16), -3.0, np.float16)
+    C = np.zeros((16, 16), np.float32)
+    assert np.all(emu.tensor_srgemm(A, B, C, "max_plus", False) == 0.0)
+    # 2048 + 1 is not a half: the sum rounds (to even) before the float min.
+    A = np.full((16, 16), 2048.0, np.float16)
+    B = np.full((16, 16), 1.0, np.float16)
+    assert np.all(emu.tensor_srgemm(A, B, C, "min_plus", False) == 2048.0)
+    assert np.all(emu.tensor_srgemm(A, B, C, "max_plus", False) == 2048.0)
+    # or_and treats > 0 as true
+    A = np.eye(16, dtype=np.float16) * 0.5
+    got = emu.tensor_srgemm(A, A, C, "or_and", False)
+    assert np.array_equal(got, np.eye(16, dtype=np.float32))
+
+
+def test_knn_insertion_sort_is_stable_topk(oracle_mod):
+    """The restated modified_insertion_sort_linear equals a stable argsort:
+    k smallest per column, ties in index order."""
+    rng = np.random.default_rng(9)
+    D = rng.integers(0, 20, (300, 70)).astype(np.float32)  # many ties
+    for k in (1, 5, 16, 32):
+        dist, idx = oracle_mod.knn_insertion_sort(D, k, do_sqrt=False)
+        order = np.argsort(D, axis=0, kind="stable")[:k]
+        assert np.array_equal(idx, order.astype(np.int32)), k
+        assert np.array_equal(dist, np.take_along_axis(D, order, axis=0)), k
+    dist, idx = oracle_mod.knn_insertion_sort(D, 4, do_sqrt=True)
+    assert np.allclose(dist, np.sqrt(np.sort(D, axis=0)[:4]))
