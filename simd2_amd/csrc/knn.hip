@@ -11,10 +11,11 @@
 //    k <= 32.
 //
 // 2. The precision form (no counterpart in the reference):
-//    * prep_kernel: FP32 points -> BF16 copies (zero-padded to a multiple of
-//      32 dims) and FP32 squared norms;
+//    * prep_kernel: FP32 points -> BF16 high and low parts (zero-padded to a
+//      multiple of 32 dims) and FP32 squared norms;
 //    * cand_kernel: ||q||^2 + ||r||^2 - 2 q.r with the dot product on BF16 MFMA
-//      (v_mfma_f32_16x16x32_bf16, FP32 accumulate).  A workgroup owns 64
+//      (v_mfma_f32_16x16x32_bf16, FP32 accumulate; hi.hi + hi.lo + lo.hi, three
+//      MFMAs per k-step, for ~16 significant bits).  A workgroup owns 64
 //      queries (each wave 16, their A fragments held in registers for the whole
 //      scan) and one slice of the references, streamed through LDS in 64-row
 //      tiles.  The 64x64 distance tile goes to LDS, where each thread scans 16
@@ -55,10 +56,14 @@ __device__ __forceinline__ void sorted_insert(float (&ld)[KMAX], int (&li)[KMAX]
   float cd = d;
   int ci = idx;
   float last = thr;
+  // The new element goes to the first position holding a strictly larger
+  // distance (or the first free one); every entry after it then shifts one
+  // place, unconditionally, so equal distances keep their order.
+  bool shifting = false;
 #pragma unroll
   for (int j = 0; j < KMAX; ++j) {
     if (j < k) {
-      const bool sw = (j < n) ? (cd < ld[j]) : (j == n);
+      const bool sw = shifting || ((j < n) ? (cd < ld[j]) : (j == n));
       const float td = ld[j];
       const int ti = li[j];
       if (sw) {
@@ -66,6 +71,7 @@ __device__ __forceinline__ void sorted_insert(float (&ld)[KMAX], int (&li)[KMAX]
         li[j] = ci;
         cd = td;
         ci = ti;
+        shifting = true;
       }
       if (j == k - 1) last = ld[j];
     }
@@ -138,8 +144,9 @@ __global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// One wave per point: BF16 copy (dims padded with zeros to dpad) and the FP32
-// squared norm (wave-summed).
+// One wave per point: the FP32 coordinates split into a BF16 high part and a
+// BF16 low part (x - hi), dims zero-padded to dpad, stored [point][hi | lo],
+// and the FP32 squared norm (wave-summed).
 __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, int64_t ldp,
                                                    int n, int d, int dpad, __bf16 *Pb,
                                                    float *norm) {
@@ -150,7 +157,9 @@ __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, 
   for (int j = lane; j < dpad; j += 64) {
     const float x = j < d ? P[(int64_t)p * ldp + j] : 0.f;
     s += x * x;
-    Pb[(int64_t)p * dpad + j] = __float2bfloat16(x);
+    const __bf16 hi = __float2bfloat16(x);
+    Pb[(int64_t)p * 2 * dpad + j] = hi;
+    Pb[(int64_t)p * 2 * dpad + dpad + j] = __float2bfloat16(x - __bfloat162float(hi));
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -161,6 +170,10 @@ constexpr int CQ = 64;    // queries per workgroup
 constexpr int CR = 64;    // references per tile
 constexpr int CLIST = 16; // candidates kept per thread
 
+// q.r ~ qh.rh + qh.rl + ql.rh: three BF16 MFMAs per k-step carry ~16
+// significant bits, which the norm expansion needs (centres ~N(0, 10^2) make
+// ||q||^2 + ||r||^2 ~ 100x the neighbour distances); a single BF16 product
+// leaves errors of ~10 on distances ~250 and loses true neighbours.
 template <int DPAD>
 __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb,
                                                    const __bf16 *__restrict__ Rb,
@@ -168,12 +181,13 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
                                                    const float *__restrict__ rn, int Q, int R,
                                                    int slice, float *cand_d, int *cand_i,
                                                    int ncand) {
-  constexpr int KS = DPAD / 32;         // MFMA k-steps
-  constexpr int RLD = DPAD + 8;         // padded LDS row (bf16): +16 bytes
-  constexpr int CHUNKS = CR * DPAD / 8; // 16-byte chunks per reference tile
-  __shared__ __attribute__((aligned(16))) __bf16 Rt[2][CR][RLD];
+  constexpr int KS = DPAD / 32;             // MFMA k-steps
+  constexpr int RLD = 2 * DPAD + 8;         // LDS row: hi | lo, +16 bytes of padding
+  constexpr int CHUNKS = CR * 2 * DPAD / 8; // 16-byte chunks per reference tile
+  constexpr int PER = CHUNKS / 256;         // chunks per thread
+  __shared__ __attribute__((aligned(16))) __bf16 Rt[CR][RLD];
   __shared__ float Dt[CQ][CR + 1];
-  __shared__ float Rn[2][CR];
+  __shared__ float Rn[CR];
 
   const int tid = threadIdx.x;
   const int lane = tid % 64;
@@ -183,17 +197,22 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
   const int rb1 = min(R, rb0 + slice);
 
   // A fragments of this wave's 16 queries: lane l holds query l%16, dims
-  // 8*(l/16) .. +7 of every 32-wide k-step.
-  bf16x8 a[KS];
+  // 8*(l/16) .. +7 of every 32-wide k-step, high and low parts.
+  bf16x8 ah[KS], al[KS];
   {
     const int q = q0 + wave * 16 + lane % 16;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       if (q < Q) {
-        a[ks] = *reinterpret_cast<const bf16x8 *>(Qb + (int64_t)q * DPAD + ks * 32 + 8 * (lane / 16));
+        const __bf16 *base = Qb + (int64_t)q * 2 * DPAD + ks * 32 + 8 * (lane / 16);
+        ah[ks] = *reinterpret_cast<const bf16x8 *>(base);
+        al[ks] = *reinterpret_cast<const bf16x8 *>(base + DPAD);
       } else {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) a[ks][e] = (__bf16)0.0f;
+        for (int e = 0; e < 8; ++e) {
+          ah[ks][e] = (__bf16)0.0f;
+          al[ks][e] = (__bf16)0.0f;
+        }
       }
     }
   }
@@ -217,28 +236,39 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
   int lpos = 0;
   const int sq = tid / 4, sh = tid % 4;
 
-  auto load_tile = [&](int buf, int r0) {
+  // Register prefetch of the next reference tile (global -> registers while
+  // the current tile is multiplied and scanned, registers -> LDS after).
+  uint4 pre[PER];
+  float pren = 0.f;
+  auto fetch = [&](int r0) {
 #pragma unroll
-    for (int h = 0; h < CHUNKS / 256; ++h) {
+    for (int h = 0; h < PER; ++h) {
       const int f = tid + 256 * h;
-      const int rr = f / (DPAD / 8);
-      const int cc = (f % (DPAD / 8)) * 8;
+      const int rr = f / (2 * DPAD / 8);
+      const int cc = (f % (2 * DPAD / 8)) * 8;
       const int r = r0 + rr;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (r < rb1) v = *reinterpret_cast<const uint4 *>(Rb + (int64_t)r * DPAD + cc);
-      *reinterpret_cast<uint4 *>(&Rt[buf][rr][cc]) = v;
+      pre[h] = make_uint4(0u, 0u, 0u, 0u);
+      if (r < rb1) pre[h] = *reinterpret_cast<const uint4 *>(Rb + (int64_t)r * 2 * DPAD + cc);
     }
-    if (tid < CR) {
-      const int r = r0 + tid;
-      Rn[buf][tid] = r < rb1 ? rn[r] : 0.f;
+    if (tid < CR) pren = (r0 + tid < rb1) ? rn[r0 + tid] : 0.f;
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int h = 0; h < PER; ++h) {
+      const int f = tid + 256 * h;
+      const int rr = f / (2 * DPAD / 8);
+      const int cc = (f % (2 * DPAD / 8)) * 8;
+      *reinterpret_cast<uint4 *>(&Rt[rr][cc]) = pre[h];
     }
+    if (tid < CR) Rn[tid] = pren;
   };
 
-  int buf = 0;
-  if (rb0 < rb1) load_tile(0, rb0);
-  __syncthreads();
+  if (rb0 < rb1) fetch(rb0);
   for (int r0 = rb0; r0 < rb1; r0 += CR) {
-    if (r0 + CR < rb1) load_tile(buf ^ 1, r0 + CR);
+    __syncthreads();  // previous tile fully multiplied and scanned
+    stage();
+    __syncthreads();
+    if (r0 + CR < rb1) fetch(r0 + CR);
     f32x4 acc[4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) acc[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -246,8 +276,12 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
     for (int ks = 0; ks < KS; ++ks) {
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
-        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Rt[buf][rb * 16 + lane % 16][ks * 32 + 8 * (lane / 16)]);
-        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], b, acc[rb], 0, 0, 0);
+        const __bf16 *row = &Rt[rb * 16 + lane % 16][ks * 32 + 8 * (lane / 16)];
+        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(row);
+        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(row + DPAD);
+        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, acc[rb], 0, 0, 0);
+        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, acc[rb], 0, 0, 0);
       }
     }
     // Approximate distances into the LDS tile; references past the slice end
@@ -256,7 +290,7 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
     for (int rb = 0; rb < 4; ++rb) {
       const int rr = rb * 16 + lane % 16;
       const bool valid = r0 + rr < rb1;
-      const float rnv = Rn[buf][rr];
+      const float rnv = Rn[rr];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float dd = valid ? qnv[v] + rnv - 2.0f * acc[rb][v] : __builtin_inff();
@@ -285,8 +319,6 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
           }
       }
     }
-    __syncthreads();
-    buf ^= 1;
   }
   const int q = q0 + sq;
   if (q < Q) {
@@ -443,8 +475,8 @@ size_t knn_bf16_workspace(int Q, int R, int d, int *slices_out) {
   if (slices_out) *slices_out = slices;
   const size_t ncand = (size_t)slices * 4 * CLIST;
   size_t bytes = 0;
-  bytes += ((size_t)Q * dpad * 2 + 255) / 256 * 256;
-  bytes += ((size_t)R * dpad * 2 + 255) / 256 * 256;
+  bytes += ((size_t)Q * dpad * 4 + 255) / 256 * 256;  // hi | lo BF16
+  bytes += ((size_t)R * dpad * 4 + 255) / 256 * 256;
   bytes += ((size_t)Q * 4 + 255) / 256 * 256;
   bytes += ((size_t)R * 4 + 255) / 256 * 256;
   bytes += ((size_t)Q * ncand * 4 + 255) / 256 * 256;
@@ -465,8 +497,8 @@ hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, in
     w += (bytes + 255) / 256 * 256;
     return p;
   };
-  __bf16 *Qb = reinterpret_cast<__bf16 *>(take((size_t)Q * dpad * 2));
-  __bf16 *Rb = reinterpret_cast<__bf16 *>(take((size_t)R * dpad * 2));
+  __bf16 *Qb = reinterpret_cast<__bf16 *>(take((size_t)Q * dpad * 4));
+  __bf16 *Rb = reinterpret_cast<__bf16 *>(take((size_t)R * dpad * 4));
   float *qn = reinterpret_cast<float *>(take((size_t)Q * 4));
   float *rn = reinterpret_cast<float *>(take((size_t)R * 4));
   float *cd = reinterpret_cast<float *>(take((size_t)Q * ncand * 4));
