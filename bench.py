@@ -29,9 +29,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_VECTOR_TFLOPS = 157.3   # MI355X published FP32 vector peak (FMA, packed)
-# Issue bound of the semiring inner step with packed FP32 disabled: 256 CUs x
-# 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T lane-instr/s, 3 instructions per 4 ops.
-ISA_BOUND_TOPS = 39.3216 * 4.0 / 3.0
+# VALU issue ceiling of the min_plus inner step (2 x v_add_f32 + 1 x v_min3_f32
+# per 4 semiring ops; packed FP32 is not used).  gfx950 dual-issues the adds but
+# not v_min3: a microbenchmark of that mix issues at most 1.60 wave-instructions
+# per CU per cycle, so the ceiling is 1.60 x 64 lanes x 4/3 ops x 256 CUs x 2.4 GHz.
+ISSUE_CEILING_TOPS = 1.60 * 64 * 4.0 / 3.0 * 256 * 2.4e9 / 1e12
+# Bytes moved past L2 per launch of the min_plus kernel at n = 4096 (128x64
+# tiles): rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, the factors a 48 MiB
+# streaming copy reads on the same counters (profiles/srgemm_4096_counters.md):
+# 1199 MiB fetched + 64 MiB written.
+TRAFFIC = {4096: int((1199 + 64) * 2 ** 20)}
 OPS = ["min_plus", "plus_mul", "max_plus", "max_min", "max_mul", "min_mul", "min_max",
        "or_and", "minus_square"]
 
@@ -162,6 +169,71 @@ def apsp_1024(torch, simd2_amd):
     return res
 
 
+def pld_knn(torch, simd2_amd, Q=16384, R=16384, d=128, k=16, sample=512):
+    """PLD/kNN workload: 64 Gaussian clusters (centres ~N(0, 10^2), sigma 1),
+    Q queries x R references, d = 128, k = 16.  Times the plain FP32 direct path
+    (minus_square srgemm into the Q x R matrix + the reference top-k) and the
+    precision form (BF16-MFMA candidates, FP32 re-rank, FP64 near ties), and
+    checks a sample of queries against an FP64 host computation."""
+    from simd2_amd.datasets import gaussian_clusters
+    from simd2_amd.knn import KnnBF16, knn_select
+    ref, centers = gaussian_clusters(torch, R, d, 64, seed=100)
+    qry, _ = gaussian_clusters(torch, Q, d, 64, seed=101, centers=centers)
+    qT = qry.t().contiguous()
+    D = torch.empty((R, Q), device="cuda")
+    runner = KnnBF16(Q, R, d)
+
+    def direct():
+        simd2_amd.srgemm("minus_square", ref, qT, out=D)
+        return knn_select(D, k, sqrt=False)
+
+    def precise():
+        return runner(qry, ref, k)
+
+    res = {}
+    outs = {}
+    for name, fn in (("direct_fp32", direct), ("bf16_mfma_rerank", precise)):
+        outs[name] = fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            fn()
+        e1.record()
+        e1.synchronize()
+        res[name] = {"ms": round(e0.elapsed_time(e1) / 5, 4)}
+    # FP64 truth on a sample of queries: shortlist by the FP64 expansion, then
+    # the direct FP64 sum, ordered by (distance, index).
+    rows = np.linspace(0, Q - 1, sample).astype(np.int64)
+    Q64 = qry[torch.from_numpy(rows).cuda()].double().cpu().numpy()
+    R64 = ref.double().cpu().numpy()
+    Dx = (Q64 ** 2).sum(1)[:, None] + (R64 ** 2).sum(1)[None, :] - 2.0 * (Q64 @ R64.T)
+    cand = np.argpartition(Dx, k + 8, axis=1)[:, :k + 8]
+    want_i = np.zeros((sample, k), np.int64)
+    want_d = np.zeros((sample, k))
+    for r, c in enumerate(cand):
+        dd = ((R64[c] - Q64[r]) ** 2).sum(1)
+        o = np.lexsort((c, dd))[:k]
+        want_i[r], want_d[r] = c[o], dd[o]
+    for name in outs:
+        dist, idx = outs[name]
+        if name == "direct_fp32":
+            dist, idx = dist.t(), idx.t()
+        gd = dist.cpu().numpy()[rows].astype(np.float64)
+        gi = idx.cpu().numpy()[rows]
+        res[name]["exact_topk_agreement"] = float(np.mean(np.all(gi == want_i, axis=1)))
+        res[name]["max_abs_err"] = float(np.max(np.abs(gd - want_d)))
+        res[name]["max_rel_err"] = float(np.max(np.abs(gd - want_d) / want_d))
+    st = runner.stats.cpu().tolist()
+    res["bf16_mfma_rerank"]["fp64_queries"] = st[0]
+    res["bf16_mfma_rerank"]["fp64_candidates"] = st[1]
+    res["config"] = {"Q": Q, "R": R, "d": d, "k": k, "clusters": 64,
+                     "accuracy_sample_queries": sample}
+    res["speedup_precision_form"] = round(res["direct_fp32"]["ms"] /
+                                          res["bf16_mfma_rerank"]["ms"], 2)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,14 +300,18 @@ def main():
 
     mp = per_op["min_plus"]
     mp_tops = mp["GOPS"] / 1e3
-    roofline = {"kernel": "srgemm_valu_kernel<MinPlus,128x128x16>", "bound": "valu",
+    tile = {0: "128x128", 1: "64x64", 2: "128x64"}[
+        simd2_amd.lib().simd2_pick_tile_config(0, n, n, n)]
+    roofline = {"kernel": f"srgemm_valu_kernel<MinPlus,{tile}x16>", "bound": "valu",
                 "achieved": round(mp_tops, 3), "peak": PEAK_FP32_VECTOR_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(mp_tops / PEAK_FP32_VECTOR_TFLOPS, 4),
-                "isa_bound": round(ISA_BOUND_TOPS, 2),
-                "frac_isa_bound": round(mp_tops / ISA_BOUND_TOPS, 4),
-                "traffic": None,
-                "note": "achieved = 2*4096^3 semiring ops (one (x) and one (+) per i,j,k) / "
-                        "mean launch time; unit is T semiring ops/s"}
+                "issue_ceiling": round(ISSUE_CEILING_TOPS, 2),
+                "frac_issue_ceiling": round(mp_tops / ISSUE_CEILING_TOPS, 4),
+                "traffic": TRAFFIC.get(n),
+                "note": "achieved = 2*n^3 semiring ops (one (x) and one (+) per i,j,k) / "
+                        "mean launch time on the launching stream, in T semiring ops/s; "
+                        "issue_ceiling = measured VALU issue rate of the kernel's "
+                        "instruction mix (2 dual-issuable + 1 single-issue per 4 ops)"}
     calls.clear()
     torch.cuda.empty_cache()
 
@@ -248,12 +324,15 @@ def main():
         tops = 2.0 * 8192 ** 3 / (ms * 1e-3) / 1e12
         extras["min_plus_8192"] = {"ms": round(ms, 3), "GOPS": round(tops * 1e3, 1),
                                    "frac_fp32_vector_peak": round(tops / PEAK_FP32_VECTOR_TFLOPS, 4),
-                                   "frac_isa_bound": round(tops / ISA_BOUND_TOPS, 4)}
+                                   "frac_issue_ceiling": round(tops / ISSUE_CEILING_TOPS, 4)}
         log(f"min_plus 8192^3: {ms:.3f} ms {tops:.2f} T ops/s")
         del keep, call
         torch.cuda.empty_cache()
         extras["apsp_er1024"] = apsp_1024(torch, simd2_amd)
         log(f"apsp n=1024: {extras['apsp_er1024']}")
+        extras["pld_knn"] = pld_knn(torch, simd2_amd)
+        log(f"pld knn: {extras['pld_knn']}")
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
