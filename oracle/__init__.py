@@ -46,7 +46,9 @@ _ref = None
 
 def build() -> None:
     """Compile liboracle.so (and _ref/ when the reference tree exists)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    # make's own messages go to stderr so callers' stdout (bench.py's single
+    # JSON line) stays clean
+    subprocess.run(["make", "-s", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
 
 
 def _load():
