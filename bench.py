@@ -169,6 +169,87 @@ def apsp_1024(torch, simd2_amd):
     return res
 
 
+def batched_graphs(torch, simd2_amd, count=4096, seed=0):
+    """Batched workload: `count` Erdos-Renyi graphs, n uniform in [32, 512],
+    p uniform in [0.01, 0.2], integer weights in [1, 100]; 40% min_plus APSP,
+    20% max_min MCP, 20% min_max MST (no epilogue), 20% or_and transitive
+    closure on packed bits.  Times one hipGraph solve of the whole batch
+    (default schedule, then autotuned) against the per-graph converge loop,
+    and checks every graph bit for bit against that loop."""
+    from simd2_amd.batch import Batch
+    from simd2_amd.graphs import app_graph, batch_specs
+    specs = batch_specs(count, seed)
+    mats = [app_graph(torch, op, n, p, s) for n, op, epi, check, p, s in specs]
+    torch.cuda.synchronize()
+    b = Batch([s[:4] for s in specs])
+    stream = torch.cuda.current_stream()
+
+    def upload():
+        for g, X in enumerate(mats):
+            b.upload(g, X)
+
+    def timed_solve():
+        upload()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        b.solve()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e)
+
+    timed_solve()  # first launch of the graph
+    default_ms = min(timed_solve() for _ in range(2))
+    upload()
+    t0 = time.perf_counter()
+    b.autotune()
+    tune_s = time.perf_counter() - t0
+    tuned = [timed_solve() for _ in range(3)]
+    itr, bad = b.iterations()
+    cfg, launches = b.schedule()
+    outs = [b.download(g) for g in range(count)]
+
+    # the per-graph loop (fused change test, one flag read per squaring)
+    work = {}
+    flag = torch.zeros(1, dtype=torch.int32, device="cuda")
+    singles, same_itr = [], 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (n, op, epi, check, p, s), X in zip(specs, mats):
+        Y = X.clone()
+        key = (tuple(Y.shape), Y.dtype)
+        if key not in work:
+            work[key] = torch.empty_like(Y)
+        _, it1, st = simd2_amd.converge(op, Y, epilogue=epi, check=check, work=work[key],
+                                        flag=flag)
+        singles.append((Y, it1))
+    torch.cuda.synchronize()
+    loop_ms = (time.perf_counter() - t0) * 1e3
+    identical = 0
+    for g, (Y, it1) in enumerate(singles):
+        same_itr += int(itr[g] == it1)
+        identical += int(torch.equal(outs[g].view(torch.int32), Y.view(torch.int32)))
+    ops = 0.0
+    for (n, op, epi, check, p, s), it in zip(specs, itr):
+        if op != "or_and":
+            ops += 2.0 * n ** 3 * int(it)
+    ms = float(np.median(tuned))
+    hist = {}
+    for (n, op, *_), c in zip(specs, cfg):
+        if c >= 0:
+            name = {0: "128x128", 1: "64x64", 2: "128x64"}[int(c)]
+            hist[name] = hist.get(name, 0) + 1
+    b.close()
+    return {"graphs": count, "squarings": int(itr.sum()), "max_squarings": int(itr.max()),
+            "not_converged": bad, "ms_default_schedule": round(default_ms, 3),
+            "ms_autotuned": round(ms, 3), "autotune_s": round(tune_s, 2),
+            "ms_per_graph_loop": round(loop_ms, 1),
+            "speedup_vs_loop": round(loop_ms / ms, 1),
+            "fp32_GOPS": round(ops / (ms * 1e-3) / 1e9, 1),
+            "frac_issue_ceiling": round(ops / (ms * 1e-3) / 1e12 / ISSUE_CEILING_TOPS, 4),
+            "launches_per_round": launches, "tile_configs": hist,
+            "identical_to_loop": identical, "same_iterations": same_itr}
+
+
 def pld_knn(torch, simd2_amd, Q=16384, R=16384, d=128, k=16, sample=512):
     """PLD/kNN workload: 64 Gaussian clusters (centres ~N(0, 10^2), sigma 1),
     Q queries x R references, d = 128, k = 16.  Times the plain FP32 direct path
@@ -332,6 +413,9 @@ def main():
         log(f"apsp n=1024: {extras['apsp_er1024']}")
         extras["pld_knn"] = pld_knn(torch, simd2_amd)
         log(f"pld knn: {extras['pld_knn']}")
+        torch.cuda.empty_cache()
+        extras["batched_4096"] = batched_graphs(torch, simd2_amd)
+        log(f"batched 4096 graphs: {extras['batched_4096']}")
         torch.cuda.empty_cache()
 
     cpu = None

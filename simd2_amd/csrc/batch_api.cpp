@@ -1,0 +1,490 @@
+// Host side of the batched solver: pool layout, the size-binned schedule
+// (work-item tables per (op, tile configuration)), its hipGraph capture and
+// the autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "../../include/simd2/batch.h"
+#include "../../include/simd2/converge.h"
+#include "../../include/simd2/srgemm.h"
+#include "internal.hpp"
+
+using namespace simd2;
+
+namespace {
+
+constexpr int NUM_XCD = 8;
+constexpr int BIN = 32;  // size bins of the schedule: n in (32 (b-1), 32 b]
+
+int hip_status(hipError_t e) {
+  if (e == hipSuccess) return SIMD2_SUCCESS;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return SIMD2_ERROR_NO_DEVICE;
+  return SIMD2_ERROR_INTERNAL;
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+struct Launch {
+  int op, cfg;
+  int offset, count;  // slice of the item table
+};
+
+}  // namespace
+
+struct simd2_batch {
+  int count = 0;
+  int max_rounds = 0;
+  std::vector<simd2_batch_graph> graphs;
+  std::vector<BatchGraphDev> desc;
+  std::vector<int> rows;  // allocated rows per buffer
+  std::vector<int> cfg;   // VALU tile configuration per graph (-1: or_and / empty)
+  std::vector<Launch> launches;
+  void *pool = nullptr;
+  BatchGraphDev *d_desc = nullptr;
+  int *d_state = nullptr;  // active | changed | iters, count each
+  BatchItem *d_items = nullptr;
+  size_t items_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  int *active() { return d_state; }
+  int *changed() { return d_state + count; }
+  int *iters() { return d_state + 2 * count; }
+};
+
+namespace {
+
+bool is_float_op(int op) { return op != SIMD2_OR_AND; }
+
+// Work of one tile row (or word block) of graph g under `cfg`: proportional
+// to the tile area the kernel sweeps times K.
+double tile_cost(const simd2_batch &b, int g, int cfg) {
+  const int op = b.graphs[g].op;
+  int bm, bn;
+  batch_tile_shape(op, cfg, &bm, &bn);
+  if (op == SIMD2_OR_AND) return (double)bm * bn * 32.0 * b.desc[g].cols / 32.0;
+  return (double)bm * bn * b.desc[g].kpad;
+}
+
+int tiles_of(const simd2_batch &b, int g, int cfg, int *tr, int *tc) {
+  const int op = b.graphs[g].op;
+  int bm, bn;
+  batch_tile_shape(op, cfg, &bm, &bn);
+  const int n = b.graphs[g].n;
+  *tr = (n + bm - 1) / bm;
+  *tc = op == SIMD2_OR_AND ? (b.desc[g].cols + bn - 1) / bn : (n + bn - 1) / bn;
+  return *tr * *tc;
+}
+
+// Work items of one launch: graphs largest first, each graph's tiles kept on
+// one XCD (workgroup i runs on XCD i % 8), XCDs balanced by work.  Slots an
+// XCD has no work for hold g = -1 and exit at once.
+void build_items(const simd2_batch &b, const std::vector<int> &gs, int cfg,
+                 std::vector<BatchItem> &out) {
+  std::vector<int> order(gs);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return b.graphs[x].n > b.graphs[y].n; });
+  std::vector<std::vector<BatchItem>> lists(NUM_XCD);
+  double load[NUM_XCD] = {0};
+  for (int g : order) {
+    int tr, tc;
+    const int nt = tiles_of(b, g, cfg, &tr, &tc);
+    const int x = (int)(std::min_element(load, load + NUM_XCD) - load);
+    for (int r = 0; r < tr; ++r)
+      for (int c = 0; c < tc; ++c) lists[x].push_back({g, (r << 16) | c});
+    load[x] += nt * tile_cost(b, g, cfg);
+  }
+  size_t len = 0;
+  for (auto &l : lists) len = std::max(len, l.size());
+  for (size_t j = 0; j < len; ++j)
+    for (int x = 0; x < NUM_XCD; ++x)
+      out.push_back(j < lists[x].size() ? lists[x][j] : BatchItem{-1, 0});
+}
+
+// Groups graphs by (op, cfg) into launches (largest total work first) and
+// fills `items`.  FP32 graphs with cfg < 0 are left out (the autotuner plans
+// one size bin at a time), or_and graphs unless with_orand.
+void plan(const simd2_batch &b, const std::vector<int> &cfg, bool with_orand,
+          std::vector<Launch> &launches, std::vector<BatchItem> &items) {
+  std::map<std::pair<int, int>, std::vector<int>> classes;
+  for (int g = 0; g < b.count; ++g) {
+    if (b.graphs[g].n == 0) continue;
+    const bool orand = b.graphs[g].op == SIMD2_OR_AND;
+    if (orand ? with_orand : cfg[g] >= 0) classes[{b.graphs[g].op, cfg[g]}].push_back(g);
+  }
+  std::vector<std::pair<double, std::pair<int, int>>> order;
+  for (auto &kv : classes) {
+    double w = 0;
+    for (int g : kv.second) {
+      int tr, tc;
+      w += tiles_of(b, g, kv.first.second, &tr, &tc) * tile_cost(b, g, kv.first.second);
+    }
+    order.push_back({-w, kv.first});
+  }
+  std::sort(order.begin(), order.end());
+  launches.clear();
+  items.clear();
+  for (auto &o : order) {
+    Launch L;
+    L.op = o.second.first;
+    L.cfg = o.second.second;
+    L.offset = (int)items.size();
+    build_items(b, classes[o.second], L.cfg, items);
+    L.count = (int)items.size() - L.offset;
+    launches.push_back(L);
+  }
+}
+
+int upload_items(simd2_batch &b, const std::vector<BatchItem> &items) {
+  if (items.size() > b.items_cap) {
+    if (b.d_items) (void)hipFree(b.d_items);
+    b.d_items = nullptr;
+    b.items_cap = 0;
+    hipError_t e = hipMalloc(&b.d_items, items.size() * sizeof(BatchItem));
+    if (e != hipSuccess) return hip_status(e);
+    b.items_cap = items.size();
+  }
+  if (items.empty()) return SIMD2_SUCCESS;
+  return hip_status(hipMemcpy(b.d_items, items.data(), items.size() * sizeof(BatchItem),
+                              hipMemcpyHostToDevice));
+}
+
+// Launches of one round (r) of the current plan on stream s.
+hipError_t launch_round(simd2_batch &b, int r, hipStream_t s) {
+  for (const Launch &L : b.launches) {
+    hipError_t e = launch_batch_round(L.op, L.cfg, b.d_items + L.offset, L.count, b.d_desc,
+                                      b.pool, b.active(), b.changed(), r & 1, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+void destroy_graph(simd2_batch &b) {
+  if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.exec = nullptr;
+  b.graph = nullptr;
+}
+
+// Installs cfg as the schedule and captures reset, max_rounds rounds and the
+// final copy-back as one graph.
+int install(simd2_batch &b, const std::vector<int> &cfg) {
+  std::vector<Launch> launches;
+  std::vector<BatchItem> items;
+  plan(b, cfg, true, launches, items);
+  int st = upload_items(b, items);
+  if (st != SIMD2_SUCCESS) return st;
+  b.cfg = cfg;
+  b.launches = launches;
+  destroy_graph(b);
+  if (b.count == 0) return SIMD2_SUCCESS;
+  hipStream_t cs;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_status(e);
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    hipError_t le = launch_batch_reset(b.d_desc, b.active(), b.changed(), b.iters(), b.count, cs);
+    for (int r = 0; r < b.max_rounds && le == hipSuccess; ++r) {
+      le = launch_round(b, r, cs);
+      if (le == hipSuccess)
+        le = launch_batch_update(b.active(), b.changed(), b.iters(), b.count, r,
+                                 r == b.max_rounds - 1, cs);
+    }
+    if (le == hipSuccess) le = launch_batch_finalize(b.d_desc, b.pool, b.iters(), b.count, cs);
+    e = hipStreamEndCapture(cs, &b.graph);
+    if (le != hipSuccess) e = le;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess) {
+    destroy_graph(b);
+    return hip_status(e);
+  }
+  return SIMD2_SUCCESS;
+}
+
+// Default tile configuration for an n-vertex graph: the shape that wastes the
+// least padded work, preferring larger tiles (fewer LDS reads per op) on ties
+// within 10%.
+int default_cfg(int n) {
+  if (n <= 64) return CFG_S64;
+  const double w64 = (double)round_up(n, 64) * round_up(n, 64);
+  const double w128 = (double)round_up(n, 128) * round_up(n, 128);
+  const double w12864 = (double)round_up(n, 128) * round_up(n, 64);
+  if (w128 <= 1.10 * w64) return CFG_L128;
+  if (w12864 <= 1.05 * w64) return CFG_M128x64;
+  return CFG_S64;
+}
+
+bool cfg_allowed(int n, int cfg) { return n > 64 || cfg == CFG_S64; }
+
+}  // namespace
+
+extern "C" {
+
+int simd2_batch_create(const simd2_batch_graph *graphs, int count, int max_rounds,
+                       simd2_batch **out) {
+  if (!out) return SIMD2_ERROR_NULL_POINTER;
+  *out = nullptr;
+  if (count < 0 || max_rounds < 0) return SIMD2_ERROR_INVALID_PROBLEM;
+  if (count > 0 && !graphs) return SIMD2_ERROR_NULL_POINTER;
+  int nmax = 0;
+  for (int g = 0; g < count; ++g) {
+    const simd2_batch_graph &G = graphs[g];
+    if (G.n < 0 || G.n > (1 << 20)) return SIMD2_ERROR_INVALID_PROBLEM;
+    if (!batch_op_supported(G.op)) return SIMD2_ERROR_NOT_SUPPORTED;
+    if (G.op != SIMD2_OR_AND && (G.check_mode < 0 || G.check_mode > 2))
+      return SIMD2_ERROR_NOT_SUPPORTED;
+    nmax = std::max(nmax, G.n);
+  }
+  simd2_batch *b = new simd2_batch;
+  b->count = count;
+  b->graphs.assign(graphs, graphs + count);
+  if (max_rounds == 0) {
+    int lg = 0;
+    while ((1 << lg) < std::max(nmax - 1, 1)) ++lg;
+    max_rounds = lg + 3;
+  }
+  b->max_rounds = max_rounds;
+  b->desc.resize(count);
+  b->rows.resize(count);
+  b->cfg.assign(count, -1);
+  int64_t total = 0;
+  for (int g = 0; g < count; ++g) {
+    const simd2_batch_graph &G = graphs[g];
+    BatchGraphDev &D = b->desc[g];
+    D.n = G.n;
+    D.epi = G.do_epilogue ? 1 : 0;
+    if (G.op == SIMD2_OR_AND) {
+      D.cols = (G.n + 31) / 32;
+      D.ld = D.cols;
+      D.kpad = G.n;
+      D.mode = CHECK_EXACT;
+      b->rows[g] = G.n;
+    } else {
+      // Rows and columns padded to the largest tile any configuration may
+      // use, so every tile and every K step the kernel touches exists.
+      const int P = G.n <= 64 ? (int)round_up(G.n, 64) : (int)round_up(G.n, 128);
+      D.cols = G.n;
+      D.ld = P;
+      D.kpad = (int)round_up(G.n, 16);
+      D.mode = G.check_mode;
+      b->rows[g] = P;
+      if (G.n > 0) b->cfg[g] = default_cfg(G.n);
+    }
+    if (G.n == 0) b->rows[g] = 0;
+    const int64_t sz = round_up((int64_t)b->rows[g] * D.ld, 64);  // 256-byte aligned
+    D.off[0] = total;
+    D.off[1] = total + sz;
+    total += 2 * sz;
+  }
+  hipError_t e = hipSuccess;
+  if (total > 0) e = hipMalloc(&b->pool, (size_t)total * 4);
+  if (e == hipSuccess && count > 0)
+    e = hipMalloc(&b->d_desc, sizeof(BatchGraphDev) * count);
+  if (e == hipSuccess && count > 0) e = hipMalloc(&b->d_state, sizeof(int) * 3 * count);
+  if (e == hipSuccess && count > 0)
+    e = hipMemcpy(b->d_desc, b->desc.data(), sizeof(BatchGraphDev) * count,
+                  hipMemcpyHostToDevice);
+  for (int g = 0; g < count && e == hipSuccess; ++g) {
+    if (graphs[g].n == 0) continue;
+    float px, py;
+    batch_pad_values(graphs[g].op, &px, &py);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+      e = launch_batch_pad(static_cast<uint32_t *>(b->pool) + b->desc[g].off[k], b->rows[g],
+                           b->desc[g].ld, graphs[g].n, px, py, nullptr);
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  int st = hip_status(e);
+  if (st == SIMD2_SUCCESS) st = install(*b, b->cfg);
+  if (st != SIMD2_SUCCESS) {
+    simd2_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return SIMD2_SUCCESS;
+}
+
+int simd2_batch_destroy(simd2_batch *b) {
+  if (!b) return SIMD2_SUCCESS;
+  destroy_graph(*b);
+  if (b->pool) (void)hipFree(b->pool);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_state) (void)hipFree(b->d_state);
+  if (b->d_items) (void)hipFree(b->d_items);
+  delete b;
+  return SIMD2_SUCCESS;
+}
+
+static int copy_graph(simd2_batch *b, int g, void *host_side, int64_t ld, bool in,
+                      hipStream_t stream) {
+  if (!b) return SIMD2_ERROR_NULL_POINTER;
+  if (g < 0 || g >= b->count) return SIMD2_ERROR_INVALID_PROBLEM;
+  const BatchGraphDev &D = b->desc[g];
+  if (D.n == 0) return SIMD2_SUCCESS;
+  if (!host_side) return SIMD2_ERROR_NULL_POINTER;
+  if (ld < D.cols) return SIMD2_ERROR_INVALID_PROBLEM;
+  void *dev = static_cast<uint32_t *>(b->pool) + D.off[0];
+  const size_t row = (size_t)D.cols * 4;
+  hipError_t e = in ? hipMemcpy2DAsync(dev, (size_t)D.ld * 4, host_side, (size_t)ld * 4, row,
+                                       D.n, hipMemcpyDefault, stream)
+                    : hipMemcpy2DAsync(host_side, (size_t)ld * 4, dev, (size_t)D.ld * 4, row,
+                                       D.n, hipMemcpyDefault, stream);
+  return hip_status(e);
+}
+
+int simd2_batch_upload(simd2_batch *b, int g, const void *src, int64_t ld, hipStream_t stream) {
+  return copy_graph(b, g, const_cast<void *>(src), ld, true, stream);
+}
+
+int simd2_batch_download(simd2_batch *b, int g, void *dst, int64_t ld, hipStream_t stream) {
+  return copy_graph(b, g, dst, ld, false, stream);
+}
+
+int simd2_batch_buffer(simd2_batch *b, int g, void **ptr, int64_t *ld) {
+  if (!b || !ptr || !ld) return SIMD2_ERROR_NULL_POINTER;
+  if (g < 0 || g >= b->count) return SIMD2_ERROR_INVALID_PROBLEM;
+  *ptr = b->desc[g].n ? static_cast<uint32_t *>(b->pool) + b->desc[g].off[0] : nullptr;
+  *ld = b->desc[g].ld;
+  return SIMD2_SUCCESS;
+}
+
+int simd2_batch_solve(simd2_batch *b, hipStream_t stream) {
+  if (!b) return SIMD2_ERROR_NULL_POINTER;
+  if (b->count == 0) return SIMD2_SUCCESS;
+  if (!b->exec) return SIMD2_ERROR_INTERNAL;
+  return hip_status(hipGraphLaunch(b->exec, stream));
+}
+
+int simd2_batch_iterations(simd2_batch *b, int *iters, int *unconverged, hipStream_t stream) {
+  if (!b) return SIMD2_ERROR_NULL_POINTER;
+  if (unconverged) *unconverged = 0;
+  if (b->count == 0) return SIMD2_SUCCESS;
+  std::vector<int> state(3 * (size_t)b->count);
+  hipError_t e = hipMemcpyAsync(state.data(), b->d_state, state.size() * sizeof(int),
+                                hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_status(e);
+  int bad = 0;
+  for (int g = 0; g < b->count; ++g) {
+    if (iters) iters[g] = state[2 * (size_t)b->count + g];
+    bad += state[g] != 0;  // still active after the last round
+  }
+  if (unconverged) *unconverged = bad;
+  return SIMD2_SUCCESS;
+}
+
+int simd2_batch_schedule(simd2_batch *b, int *cfg, int *launches) {
+  if (!b) return SIMD2_ERROR_NULL_POINTER;
+  if (cfg)
+    for (int g = 0; g < b->count; ++g) cfg[g] = b->cfg[g];
+  if (launches) *launches = (int)b->launches.size();
+  return SIMD2_SUCCESS;
+}
+
+// Autotuner.  Step 1: for every (op, size bin) with more than one admissible
+// tile shape, time one squaring of the bin's graphs per shape (round 0:
+// reads buffer 0, writes buffer 1) and keep the fastest.  Step 2: time a
+// whole first round for that per-bin schedule and for each uniform schedule
+// (one shape for every graph that admits it), which also prices the launch
+// count and tails; install the fastest.
+int simd2_batch_autotune(simd2_batch *b, hipStream_t stream) {
+  if (!b) return SIMD2_ERROR_NULL_POINTER;
+  if (b->count == 0) return SIMD2_SUCCESS;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return SIMD2_ERROR_INTERNAL;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return SIMD2_ERROR_INTERNAL;
+  }
+  const std::vector<int> saved_cfg = b->cfg;
+  const std::vector<Launch> saved_launches = b->launches;
+  int st = SIMD2_SUCCESS;
+
+  // Times `reps` runs of round 0 of the plan for cfg (all graphs active).
+  auto time_plan = [&](const std::vector<int> &cfg, bool with_orand, int reps,
+                       float *ms) -> int {
+    std::vector<Launch> launches;
+    std::vector<BatchItem> items;
+    plan(*b, cfg, with_orand, launches, items);
+    int s = upload_items(*b, items);
+    if (s != SIMD2_SUCCESS) return s;
+    b->launches = launches;
+    hipError_t e = launch_batch_reset(b->d_desc, b->active(), b->changed(), b->iters(),
+                                      b->count, stream);
+    if (e == hipSuccess) e = launch_round(*b, 0, stream);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_round(*b, 0, stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    *ms = t / reps;
+    return hip_status(e);
+  };
+
+  // Step 1: per (op, bin).
+  std::map<std::pair<int, int>, std::vector<int>> bins;
+  for (int g = 0; g < b->count; ++g)
+    if (b->graphs[g].n > 0 && is_float_op(b->graphs[g].op))
+      bins[{b->graphs[g].op, (b->graphs[g].n + BIN - 1) / BIN}].push_back(g);
+  std::vector<int> best = b->cfg;
+  for (auto &kv : bins) {
+    if (st != SIMD2_SUCCESS) break;
+    const std::vector<int> &gs = kv.second;
+    int nmin = 1 << 30;
+    for (int g : gs) nmin = std::min(nmin, b->graphs[g].n);
+    float best_ms = 1e30f;
+    int arg = -1;
+    for (int cfg = 0; cfg < CFG_COUNT && st == SIMD2_SUCCESS; ++cfg) {
+      if (!cfg_allowed(nmin, cfg)) continue;
+      std::vector<int> trial(b->count, -1);  // this bin's graphs only
+      for (int g : gs) trial[g] = cfg;
+      float ms = 0.f;
+      st = time_plan(trial, false, 3, &ms);
+      if (st == SIMD2_SUCCESS && ms < best_ms) {
+        best_ms = ms;
+        arg = cfg;
+      }
+    }
+    if (arg >= 0)
+      for (int g : gs) best[g] = arg;
+  }
+
+  // Step 2: whole-round comparison of the per-bin schedule and the uniform ones.
+  std::vector<std::vector<int>> candidates = {best};
+  for (int cfg = 0; cfg < CFG_COUNT; ++cfg) {
+    std::vector<int> c = b->cfg;
+    for (int g = 0; g < b->count; ++g)
+      if (c[g] >= 0) c[g] = cfg_allowed(b->graphs[g].n, cfg) ? cfg : CFG_S64;
+    candidates.push_back(c);
+  }
+  std::vector<int> chosen = best;
+  float chosen_ms = 1e30f;
+  for (auto &c : candidates) {
+    if (st != SIMD2_SUCCESS) break;
+    float ms = 0.f;
+    st = time_plan(c, true, 3, &ms);
+    if (st == SIMD2_SUCCESS && ms < chosen_ms) {
+      chosen_ms = ms;
+      chosen = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != SIMD2_SUCCESS) {
+    b->cfg = saved_cfg;
+    b->launches = saved_launches;
+    (void)install(*b, saved_cfg);
+    return st;
+  }
+  return install(*b, chosen);
+}
+
+}  // extern "C"

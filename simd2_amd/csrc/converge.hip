@@ -15,7 +15,7 @@ constexpr int EW_THREADS = 256;
 __global__ __launch_bounds__(EW_THREADS) void comp_update_f32_kernel(
     float *prev, float *cur, int *flag, int64_t n, int mode) {
   bool changed = false;
-  for (int64_t i = (int64_t)blockIdx.x * EW_THREADS + threadIdx.x; i < n;
+  for (int64_t i = (int64_t)bid_x() * EW_THREADS + tid_x(); i < n;
        i += (int64_t)gridDim.x * EW_THREADS) {
     float c = cur[i];
     const float p = prev[i];
@@ -31,24 +31,24 @@ __global__ __launch_bounds__(EW_THREADS) void comp_update_f32_kernel(
     }
     prev[i] = c;
   }
-  if (__any(changed) && (threadIdx.x % warpSize) == 0) atomicOr(flag, 1);
+  if (__any(changed) && (tid_x() % warpSize) == 0) atomicOr(flag, 1);
 }
 
 // kernel/converge.cuh:61-69.
 __global__ __launch_bounds__(EW_THREADS) void comp_update_i32_kernel(
     int *prev, const int *cur, int *flag, int64_t n) {
   bool changed = false;
-  for (int64_t i = (int64_t)blockIdx.x * EW_THREADS + threadIdx.x; i < n;
+  for (int64_t i = (int64_t)bid_x() * EW_THREADS + tid_x(); i < n;
        i += (int64_t)gridDim.x * EW_THREADS) {
     const int c = cur[i];
     changed |= (prev[i] != c);
     prev[i] = c;
   }
-  if (__any(changed) && (threadIdx.x % warpSize) == 0) atomicOr(flag, 1);
+  if (__any(changed) && (tid_x() % warpSize) == 0) atomicOr(flag, 1);
 }
 
 __global__ __launch_bounds__(EW_THREADS) void fill_i32_kernel(int *p, int v, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * EW_THREADS + threadIdx.x; i < n;
+  for (int64_t i = (int64_t)bid_x() * EW_THREADS + tid_x(); i < n;
        i += (int64_t)gridDim.x * EW_THREADS)
     p[i] = v;
 }
@@ -58,10 +58,10 @@ __global__ __launch_bounds__(EW_THREADS) void copy_2d_kernel(
     int rows, int row_bytes) {
   // One row per block-iteration, 4-byte granules (row_bytes % 4 == 0).
   const int words = row_bytes / 4;
-  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+  for (int r = bid_x(); r < rows; r += gridDim.x) {
     const uint32_t *s = reinterpret_cast<const uint32_t *>(src + (int64_t)r * lds);
     uint32_t *d = reinterpret_cast<uint32_t *>(dst + (int64_t)r * ldd);
-    for (int w = threadIdx.x; w < words; w += EW_THREADS) d[w] = s[w];
+    for (int w = tid_x(); w < words; w += EW_THREADS) d[w] = s[w];
   }
 }
 

@@ -3,6 +3,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifdef __HIP__
+#include "hw.hpp"  // kernel sources only
+#endif
+
 namespace simd2 {
 
 // Row-major problem every kernel solves:
@@ -68,6 +72,68 @@ size_t knn_bf16_workspace(int Q, int R, int d, int *slices_out);
 hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, int64_t ldr, int Q,
                            int R, int d, int k, float *out_d, int *out_i, int do_sqrt,
                            void *workspace, int *stats, hipStream_t s);
+
+// Batched repeated squaring (batch.hip).  Buffers are 4-byte elements of one
+// pool (floats, or packed 32-bit words for or_and).
+struct BatchGraphDev {
+  int64_t off[2];  // element offsets of the two buffers in the pool
+  int n;           // vertices
+  int cols;        // live elements per row: n, or ceil(n/32) words for or_and
+  int ld;          // row stride in elements
+  int kpad;        // K loop bound of the VALU kernel (multiple of 16)
+  int epi;         // 1: result (+)= current matrix
+  int mode;        // CheckMode of the change test
+};
+struct BatchItem {
+  int g;     // graph index, -1 for an idle slot
+  int tile;  // tile row << 16 | tile column (or word-block column)
+};
+bool batch_op_supported(int op);
+void batch_pad_values(int op, float *pad_x, float *pad_y);
+void batch_tile_shape(int op, int cfg, int *bm, int *bn);
+hipError_t launch_batch_round(int op, int cfg, const BatchItem *items, int n_items,
+                              const BatchGraphDev *graphs, void *pool, const int *active,
+                              int *changed, int parity, hipStream_t s);
+hipError_t launch_batch_reset(const BatchGraphDev *graphs, int *active, int *changed, int *iters,
+                              int count, hipStream_t s);
+hipError_t launch_batch_update(int *active, int *changed, int *iters, int count, int round,
+                               int last, hipStream_t s);
+hipError_t launch_batch_finalize(const BatchGraphDev *graphs, void *pool, const int *iters,
+                                 int count, hipStream_t s);
+hipError_t launch_batch_pad(void *buf, int rows, int64_t ld, int n, float pad_x, float pad_y,
+                            hipStream_t s);
+
+// Scheduled repeated squaring (sched.hip): one squaring D' = [D (+)] D (x) D
+// over padded ld x ld buffers that skips (a) tiles whose row panel and column
+// panel did not change in the previous squaring (their result is bit-for-bit
+// the previous one, already in the output buffer) and (b) K slabs that hold
+// only the (+)-identity in the tile's rows or columns.
+constexpr int SCHED_MAX_SLABS = 2048;  // n <= 32768
+struct SchedArgs {
+  const float *cur;
+  float *nxt;
+  int64_t ld;
+  int n, kpad, epi, mode;
+  int tiles_r, tiles_c, slabs;  // slab = 16 consecutive k
+  const uint8_t *xocc_in;       // [tiles_r][slabs]: D rows of tile row, 16 columns
+  uint8_t *xocc_out;
+  const uint8_t *yocc_in;       // [tiles_c][slabs]: 16 D rows, columns of tile column
+  uint8_t *yocc_out;
+  const int *rowchg_prev;       // tile rows / columns changed by the previous squaring
+  const int *colchg_prev;       //   (null: first squaring, every tile is computed)
+  int *rowchg;                  // written by this squaring (exact bit test)
+  int *colchg;
+  const int *conv_prev;         // change-test flag of the previous squaring (null: first)
+  int *conv;                    // change-test flag (check mode) of this squaring
+  unsigned long long *stats;    // optional: [0] tiles computed, [1] slab-tiles computed
+};
+hipError_t launch_sched_square(int op, int cfg, const SchedArgs &a, hipStream_t s);
+hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, int n,
+                                  int tiles_r, int tiles_c, int slabs, uint8_t *xocc,
+                                  uint8_t *yocc, hipStream_t s);
+hipError_t launch_fill_f32(float *p, float v, int64_t n, hipStream_t s);
+hipError_t launch_permute_copy(float *dst, int64_t ldd, const float *src, int64_t lds, int n,
+                               const int *perm, int inverse, hipStream_t s);
 
 // Element-wise helpers (converge.hip).
 hipError_t launch_comp_update_f32(float *prev, float *cur, int *flag,

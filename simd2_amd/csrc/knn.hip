@@ -86,9 +86,9 @@ __global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
   __shared__ float sd[SEL_PARTS][KMAX][SEL_Q];
   __shared__ int si[SEL_PARTS][KMAX][SEL_Q];
   __shared__ int sn[SEL_PARTS][SEL_Q];
-  const int lane = threadIdx.x % 64;
-  const int part = threadIdx.x / 64;
-  const int q = blockIdx.x * SEL_Q + lane;
+  const int lane = tid_x() % 64;
+  const int part = tid_x() / 64;
+  const int q = bid_x() * SEL_Q + lane;
   const int chunk = (R + SEL_PARTS - 1) / SEL_PARTS;
   const int r0 = part * chunk;
   const int r1 = min(R, r0 + chunk);
@@ -150,8 +150,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, int64_t ldp,
                                                    int n, int d, int dpad, __bf16 *Pb,
                                                    float *norm) {
-  const int lane = threadIdx.x % 64;
-  const int p = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = tid_x() % 64;
+  const int p = bid_x() * 4 + tid_x() / 64;
   if (p >= n) return;
   float s = 0.f;
   for (int j = lane; j < dpad; j += 64) {
@@ -189,11 +189,11 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
   __shared__ float Dt[CQ][CR + 1];
   __shared__ float Rn[CR];
 
-  const int tid = threadIdx.x;
+  const int tid = tid_x();
   const int lane = tid % 64;
   const int wave = tid / 64;
-  const int q0 = blockIdx.x * CQ;
-  const int rb0 = blockIdx.y * slice;
+  const int q0 = bid_x() * CQ;
+  const int rb0 = bid_y() * slice;
   const int rb1 = min(R, rb0 + slice);
 
   // A fragments of this wave's 16 queries: lane l holds query l%16, dims
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
   }
   const int q = q0 + sq;
   if (q < Q) {
-    const int64_t base = (int64_t)q * ncand + (int64_t)blockIdx.y * (4 * CLIST) + sh * CLIST;
+    const int64_t base = (int64_t)q * ncand + (int64_t)bid_y() * (4 * CLIST) + sh * CLIST;
 #pragma unroll
     for (int c = 0; c < CLIST; ++c) {
       cand_d[base + c] = ld[c];
@@ -348,8 +348,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float *__restrict__ Q
                                                      const int *__restrict__ cand_i, int ncand,
                                                      float *out_d, int *out_i, int do_sqrt,
                                                      int *stats) {
-  const int lane = threadIdx.x % 64;
-  const int q = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = tid_x() % 64;
+  const int q = bid_x() * 4 + tid_x() / 64;
   if (q >= Q) return;
   const float *cd = cand_d + (int64_t)q * ncand;
   const int *ci = cand_i + (int64_t)q * ncand;

@@ -25,12 +25,12 @@ __global__ __launch_bounds__(256) void plusmul_mfma_kernel(
     const float *C, int64_t ldc, int tiles_c) {
   __shared__ __attribute__((aligned(16))) float Xs[2][KB][MB + LDP];
   __shared__ __attribute__((aligned(16))) float Ys[2][KB][NB + LDP];
-  const int tid = threadIdx.x;
+  const int tid = tid_x();
   const int lane = tid % 64;
   const int wave = tid / 64;
   const int wr = wave / 2, wc = wave % 2;
-  const int row0 = (blockIdx.x / tiles_c) * MB;
-  const int col0 = (blockIdx.x % tiles_c) * NB;
+  const int row0 = (bid_x() / tiles_c) * MB;
+  const int col0 = (bid_x() % tiles_c) * NB;
 
   f32x4 acc[4][4];
 #pragma unroll

@@ -44,11 +44,11 @@ __global__ __launch_bounds__(256) void orand_packed_kernel(
     int64_t ldc, uint32_t *D, int64_t ldd, const uint32_t *P, int64_t ldp,
     int *flag, uint32_t last_mask, int tiles_w) {
   __shared__ __attribute__((aligned(16))) uint32_t Bs[PK_BK][PK_BW];
-  const int tid = threadIdx.x;
+  const int tid = tid_x();
   const int tw = tid % PK_NTW;
   const int trr = tid / PK_NTW;
-  const int tile_r = blockIdx.x / tiles_w;
-  const int tile_w = blockIdx.x % tiles_w;
+  const int tile_r = bid_x() / tiles_w;
+  const int tile_w = bid_x() % tiles_w;
   const int row0 = tile_r * PK_BM + trr * PK_TR;
   const int w0 = tile_w * PK_BW;
 
@@ -128,11 +128,11 @@ __global__ __launch_bounds__(256) void orand_i32_kernel(
     int *O, int64_t ldo, const int *P, int64_t ldp, int *flag, int tiles_c) {
   __shared__ uint32_t Xp[IB_BM][IB_BKW + 1];   // rows packed along k
   __shared__ uint32_t Yp[IB_BN][IB_BKW + 1];   // columns packed along k
-  const int tid = threadIdx.x;
+  const int tid = tid_x();
   const int lane = tid % 64;
   const int wave = tid / 64;
-  const int row0 = (blockIdx.x / tiles_c) * IB_BM;
-  const int col0 = (blockIdx.x % tiles_c) * IB_BN;
+  const int row0 = (bid_x() / tiles_c) * IB_BM;
+  const int col0 = (bid_x() % tiles_c) * IB_BN;
   const int tr = tid / 16;  // 16 x 16 threads, 4 x 4 outputs each
   const int tc = tid % 16;
 

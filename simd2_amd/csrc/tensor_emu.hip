@@ -66,9 +66,9 @@ __global__ __launch_bounds__(256) void tensor_emu_kernel(const __half *__restric
                                                          int n, int k, int do_epilogue) {
   __shared__ __half As[TE][TE + 2];
   __shared__ __half Bs[TE][TE + 2];
-  const int tx = threadIdx.x % TE;   // column within the tile
-  const int ty = threadIdx.x / TE;   // row within the tile
-  const int row0 = blockIdx.y * TE, col0 = blockIdx.x * TE;
+  const int tx = tid_x() % TE;   // column within the tile
+  const int ty = tid_x() / TE;   // row within the tile
+  const int row0 = bid_y() * TE, col0 = bid_x() * TE;
   float acc = emu_init(OP);
   for (int k0 = 0; k0 < k; k0 += TE) {
     As[ty][tx] = a[(int64_t)(row0 + ty) * k + k0 + tx];
@@ -83,13 +83,13 @@ __global__ __launch_bounds__(256) void tensor_emu_kernel(const __half *__restric
 }
 
 __global__ void f2h_kernel(const float *in, __half *out, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+  for (int64_t i = (int64_t)bid_x() * blockDim.x + tid_x(); i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     out[i] = __float2half(in[i]);
 }
 
 __global__ void h2f_kernel(const __half *in, float *out, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+  for (int64_t i = (int64_t)bid_x() * blockDim.x + tid_x(); i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     out[i] = __half2float(in[i]);
 }

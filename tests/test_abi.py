@@ -12,7 +12,7 @@ from conftest import ROOT
 import simd2_amd
 
 HEADERS = [os.path.join(ROOT, "include", "simd2", h)
-           for h in ("srgemm.h", "converge.h", "tensor_emu.h", "knn.h")]
+           for h in ("srgemm.h", "converge.h", "tensor_emu.h", "knn.h", "batch.h")]
 
 
 def declared_functions():
@@ -122,3 +122,30 @@ def test_knn_argument_errors(lib):
                             None) == simd2_amd.ERROR_INVALID_PROBLEM   # dim > 160
     assert L.simd2_knn_bf16(P1, 128, P2, 128, 1000, 2000, 128, 32, P3, P4, 0, P4, ws, None,
                             None) == simd2_amd.ERROR_INVALID_PROBLEM   # k > 31
+
+
+def test_batch_argument_errors(lib):
+    import simd2_amd.batch as B
+    L = B._bind()
+    h = ctypes.c_void_p()
+
+    def create(specs, max_rounds=0):
+        arr = (B._Graph * len(specs))(*[B._Graph(*s) for s in specs])
+        return L.simd2_batch_create(arr, len(specs), max_rounds, ctypes.byref(h))
+
+    assert L.simd2_batch_create(None, 1, 0, ctypes.byref(h)) == simd2_amd.ERROR_NULL_POINTER
+    assert L.simd2_batch_create(None, 0, 0, None) == simd2_amd.ERROR_NULL_POINTER
+    assert L.simd2_batch_create(None, -1, 0, ctypes.byref(h)) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert create([(8, 0, 1, 1)], -1) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert create([(-1, 0, 1, 1)]) == simd2_amd.ERROR_INVALID_PROBLEM
+    # plus_mul and minus_square have no fixpoint loop; unknown ops and modes
+    for op in (1, 8, 9, -1):
+        assert create([(8, 0, 1, 1), (8, op, 1, 1)]) == simd2_amd.ERROR_NOT_SUPPORTED
+    assert create([(8, 0, 1, 3)]) == simd2_amd.ERROR_NOT_SUPPORTED
+    assert h.value is None
+    # every other entry point rejects a null batch before touching the GPU
+    assert L.simd2_batch_solve(None, None) == simd2_amd.ERROR_NULL_POINTER
+    assert L.simd2_batch_autotune(None, None) == simd2_amd.ERROR_NULL_POINTER
+    assert L.simd2_batch_upload(None, 0, P1, 8, None) == simd2_amd.ERROR_NULL_POINTER
+    assert L.simd2_batch_iterations(None, None, None, None) == simd2_amd.ERROR_NULL_POINTER
+    assert L.simd2_batch_destroy(None) == simd2_amd.SUCCESS
