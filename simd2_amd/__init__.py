@@ -70,6 +70,10 @@ EXPORTS = list(DROPIN.values()) + [
     "simd2_batch_create", "simd2_batch_destroy", "simd2_batch_upload", "simd2_batch_download",
     "simd2_batch_buffer", "simd2_batch_autotune", "simd2_batch_solve",
     "simd2_batch_iterations", "simd2_batch_schedule",
+    "simd2_converge_scheduled_workspace", "simd2_converge_scheduled",
+    "simd2_graph_features_workspace", "simd2_graph_features", "simd2_graph_order",
+    "simd2_mlp_forward", "simd2_mlp_workspace", "simd2_mlp_gradient", "simd2_mlp_train_step",
+    "simd2_converge_learned_workspace", "simd2_converge_learned",
 ]
 
 

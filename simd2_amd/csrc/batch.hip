@@ -5,14 +5,19 @@
 // the VALU tile body runs with 128-bit loads everywhere and only its stores
 // are masked to the live n x n block (valu_tile MASKED mode).
 //
-// A round is a handful of launches, one per (op, tile configuration), each
-// over a static table of (graph, tile) work items built on the host: items
-// of one graph sit on one XCD (blockIdx % 8), graphs are balanced across the
-// XCDs and issued largest first.  A workgroup whose graph has already
-// converged returns at once.  After the launches a tiny update kernel retires
-// every graph whose fused change flag stayed clear.  Rounds are level
-// synchronous (kernel boundaries order them), so no workgroup ever waits on
-// another and the whole solve can be captured as one hipGraph.
+// A round is a handful of launches, one per (op, tile configuration) class,
+// over a static table of (graph, tile) work items built on the host: one list
+// per XCD (all tiles of a graph in one list, so its panels stay in one L2;
+// graphs balanced across the lists, largest first), and workgroup b takes
+// item b / 8 of list b % 8 -- the hardware dispatches workgroups to the XCDs
+// round-robin and in order.  Items of graphs that have converged return at
+// once; a launch whose whole class has converged returns at once.  After the
+// launches a tiny update kernel retires every graph whose fused change flag
+// stayed clear.  Rounds are level synchronous (kernel boundaries order them)
+// and no workgroup ever waits on another, so the whole solve is captured as
+// one hipGraph.  (Workgroups that stay resident and claim items through
+// atomic counters measured about 10% slower on the 4096-graph workload: the
+// counters serialise in L2.)
 #include <algorithm>
 #include <cstring>
 
@@ -24,15 +29,28 @@ namespace {
 
 constexpr int EW = 256;
 
+// Work item of workgroup b: the (b / 8)-th item of the list of XCD b % 8
+// (workgroups are dispatched to the XCDs round-robin and in order, so each
+// XCD works through its own list, largest graphs first); -1 past its end.
+__device__ __forceinline__ int item_of_group(const BatchLaunchDev *__restrict__ L) {
+  const int b = bid_x();
+  const int x = b % BATCH_XCDS, j = b / BATCH_XCDS;
+  return j < L->len[x] ? L->off[x] + j : -1;
+}
+
 template <class Op, class Cfg>
 __global__ __launch_bounds__((Cfg::BM / Cfg::TM) * (Cfg::BN / Cfg::TN))
 __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void batch_valu_kernel(
-    const BatchItem *__restrict__ items, const BatchGraphDev *__restrict__ graphs,
-    float *pool, const int *__restrict__ active, int *changed, int parity) {
+    const BatchItem *__restrict__ items, const BatchLaunchDev *__restrict__ L,
+    const BatchGraphDev *__restrict__ graphs, float *pool, const int *__restrict__ active,
+    int *changed, int parity, const int *class_active) {
   __shared__ ValuSmem<Cfg::BM, Cfg::BN, Cfg::BK> sm;
-  const int g = __builtin_amdgcn_readfirstlane(items[bid_x()].g);
-  const int tile = __builtin_amdgcn_readfirstlane(items[bid_x()].tile);
-  if (g < 0 || !active[g]) return;  // uniform over the workgroup
+  if (class_active && *class_active == 0) return;  // every graph of the class has converged
+  const int idx = __builtin_amdgcn_readfirstlane(item_of_group(L));
+  if (idx < 0) return;
+  const int g = __builtin_amdgcn_readfirstlane(items[idx].g);
+  const int tile = __builtin_amdgcn_readfirstlane(items[idx].tile);
+  if (!active[g]) return;  // uniform over the workgroup
   const BatchGraphDev G = graphs[g];
   const float *cur = pool + (parity ? G.off[1] : G.off[0]);
   float *nxt = pool + (parity ? G.off[0] : G.off[1]);
@@ -57,14 +75,28 @@ constexpr int BO_TR = 4, BO_NTR = 16, BO_NTW = 16;
 constexpr int BO_BM = BO_TR * BO_NTR;  // 64 rows
 constexpr int BO_BW = BO_NTW;          // 16 words
 
+__device__ __forceinline__ void orand_item(const BatchGraphDev &G, int g, int tile,
+                                           uint32_t *pool, int *changed, int parity,
+                                           uint32_t (&Bs)[32][BO_BW]);
+
 __global__ __launch_bounds__(256) void batch_orand_kernel(
-    const BatchItem *__restrict__ items, const BatchGraphDev *__restrict__ graphs,
-    uint32_t *pool, const int *__restrict__ active, int *changed, int parity) {
+    const BatchItem *__restrict__ items, const BatchLaunchDev *__restrict__ L,
+    const BatchGraphDev *__restrict__ graphs, uint32_t *pool, const int *__restrict__ active,
+    int *changed, int parity, const int *class_active) {
   __shared__ uint32_t Bs[32][BO_BW];
-  const int g = __builtin_amdgcn_readfirstlane(items[bid_x()].g);
-  const int tile = __builtin_amdgcn_readfirstlane(items[bid_x()].tile);
-  if (g < 0 || !active[g]) return;
-  const BatchGraphDev G = graphs[g];
+  if (class_active && *class_active == 0) return;
+  const int idx = __builtin_amdgcn_readfirstlane(item_of_group(L));
+  if (idx < 0) return;
+  const int g = __builtin_amdgcn_readfirstlane(items[idx].g);
+  const int tile = __builtin_amdgcn_readfirstlane(items[idx].tile);
+  if (!active[g]) return;
+  orand_item(graphs[g], g, tile, pool, changed, parity, Bs);
+}
+
+// One 64-row x 16-word tile of a packed or_and squaring.
+__device__ __forceinline__ void orand_item(const BatchGraphDev &G, int g, int tile,
+                                           uint32_t *pool, int *changed, int parity,
+                                           uint32_t (&Bs)[32][BO_BW]) {
   const uint32_t *cur = pool + (parity ? G.off[1] : G.off[0]);
   uint32_t *nxt = pool + (parity ? G.off[0] : G.off[1]);
   const int n = G.n, nw = G.cols;
@@ -138,16 +170,19 @@ __global__ __launch_bounds__(EW) void batch_reset_kernel(const BatchGraphDev *gr
 // End of round `round` (0-based): a graph whose squaring changed nothing is
 // done after round + 1 squarings; on the last round a graph that still
 // changed keeps active = 1, which the host reports as not converged.
-__global__ __launch_bounds__(EW) void batch_update_kernel(int *active, int *changed, int *iters,
-                                                          int count, int round, int last) {
+__global__ __launch_bounds__(EW) void batch_update_kernel(const BatchGraphDev *graphs,
+                                                          int *active, int *changed, int *iters,
+                                                          int *class_active_next, int count,
+                                                          int round, int last) {
   const int g = bid_x() * EW + tid_x();
   if (g >= count) return;
   if (active[g]) {
     if (!changed[g]) {
       active[g] = 0;
       iters[g] = round + 1;
-    } else if (last) {
-      iters[g] = round + 1;
+    } else {
+      if (last) iters[g] = round + 1;
+      atomicAdd(class_active_next + graphs[g].cls, 1);
     }
   }
   changed[g] = 0;
@@ -162,11 +197,17 @@ __global__ __launch_bounds__(EW) void batch_finalize_kernel(const BatchGraphDev 
   const BatchGraphDev G = graphs[g];
   const uint32_t *src = pool + G.off[1];
   uint32_t *dst = pool + G.off[0];
-  const int64_t total = (int64_t)G.n * G.cols;
-  for (int64_t i = (int64_t)bid_y() * EW + tid_x(); i < total;
-       i += (int64_t)gridDim.y * EW) {
-    const int64_t r = i / G.cols, c = i % G.cols;
-    dst[r * G.ld + c] = src[r * G.ld + c];
+  // Rows 0 .. n-1 are contiguous in both buffers and their padding is equal,
+  // so the copy is one contiguous range.
+  const int64_t total = (int64_t)G.n * G.ld;
+  if (G.ld % 4 == 0) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+    for (int64_t i = (int64_t)bid_y() * EW + tid_x(); i < total / 4; i += (int64_t)gridDim.y * EW)
+      d4[i] = s4[i];
+  } else {
+    for (int64_t i = (int64_t)bid_y() * EW + tid_x(); i < total; i += (int64_t)gridDim.y * EW)
+      dst[i] = src[i];
   }
 }
 
@@ -182,19 +223,19 @@ __global__ __launch_bounds__(EW) void batch_pad_kernel(uint32_t *buf, int rows, 
 }
 
 template <class Op>
-hipError_t launch_op_cfg(int cfg, const BatchItem *items, int n_items,
+hipError_t launch_op_cfg(int cfg, int grid, const BatchItem *items, const BatchLaunchDev *L,
                          const BatchGraphDev *graphs, float *pool, const int *active,
-                         int *changed, int parity, hipStream_t s) {
-  const dim3 grid(n_items), block(256);
+                         int *changed, int parity, const int *class_active, hipStream_t s) {
+  const dim3 g(grid), block(256);
   if (cfg == CFG_S64)
-    batch_valu_kernel<Op, CfgS><<<grid, block, 0, s>>>(items, graphs, pool, active, changed,
-                                                       parity);
+    batch_valu_kernel<Op, CfgS><<<g, block, 0, s>>>(items, L, graphs, pool, active, changed,
+                                                    parity, class_active);
   else if (cfg == CFG_M128x64)
-    batch_valu_kernel<Op, CfgM><<<grid, block, 0, s>>>(items, graphs, pool, active, changed,
-                                                       parity);
+    batch_valu_kernel<Op, CfgM><<<g, block, 0, s>>>(items, L, graphs, pool, active, changed,
+                                                    parity, class_active);
   else
-    batch_valu_kernel<Op, CfgL><<<grid, block, 0, s>>>(items, graphs, pool, active, changed,
-                                                       parity);
+    batch_valu_kernel<Op, CfgL><<<g, block, 0, s>>>(items, L, graphs, pool, active, changed,
+                                                    parity, class_active);
   return hipGetLastError();
 }
 
@@ -216,24 +257,31 @@ void batch_pad_values(int op, float *pad_x, float *pad_y) {
   }
 }
 
-hipError_t launch_batch_round(int op, int cfg, const BatchItem *items, int n_items,
-                              const BatchGraphDev *graphs, void *pool, const int *active,
-                              int *changed, int parity, hipStream_t s) {
-  if (n_items <= 0) return hipSuccess;
+hipError_t launch_batch_round(int op, int cfg, int grid, const BatchItem *items,
+                              const BatchLaunchDev *L, const BatchGraphDev *graphs, void *pool,
+                              const int *active, int *changed, int parity,
+                              const int *class_active, hipStream_t s) {
+  if (grid <= 0) return hipSuccess;
   float *fp = static_cast<float *>(pool);
+#define SIMD2_BATCH_OP(code, Op)                                                             \
+  case code:                                                                                 \
+    return launch_op_cfg<Op>(cfg, grid, items, L, graphs, fp, active, changed, parity,      \
+                             class_active, s);
   switch (op) {
-  case 0: return launch_op_cfg<MinPlus>(cfg, items, n_items, graphs, fp, active, changed, parity, s);
-  case 2: return launch_op_cfg<MaxPlus>(cfg, items, n_items, graphs, fp, active, changed, parity, s);
-  case 3: return launch_op_cfg<MaxMin>(cfg, items, n_items, graphs, fp, active, changed, parity, s);
-  case 4: return launch_op_cfg<MaxMul>(cfg, items, n_items, graphs, fp, active, changed, parity, s);
-  case 5: return launch_op_cfg<MinMul>(cfg, items, n_items, graphs, fp, active, changed, parity, s);
-  case 6: return launch_op_cfg<MinMax>(cfg, items, n_items, graphs, fp, active, changed, parity, s);
+    SIMD2_BATCH_OP(0, MinPlus)
+    SIMD2_BATCH_OP(2, MaxPlus)
+    SIMD2_BATCH_OP(3, MaxMin)
+    SIMD2_BATCH_OP(4, MaxMul)
+    SIMD2_BATCH_OP(5, MinMul)
+    SIMD2_BATCH_OP(6, MinMax)
   case 7:
-    batch_orand_kernel<<<dim3(n_items), dim3(256), 0, s>>>(
-        items, graphs, static_cast<uint32_t *>(pool), active, changed, parity);
+    batch_orand_kernel<<<dim3(grid), dim3(256), 0, s>>>(items, L, graphs,
+                                                        static_cast<uint32_t *>(pool), active,
+                                                        changed, parity, class_active);
     return hipGetLastError();
   default: return hipErrorInvalidValue;
   }
+#undef SIMD2_BATCH_OP
 }
 
 void batch_tile_shape(int op, int cfg, int *bm, int *bn) {
@@ -254,10 +302,11 @@ hipError_t launch_batch_reset(const BatchGraphDev *graphs, int *active, int *cha
   return hipGetLastError();
 }
 
-hipError_t launch_batch_update(int *active, int *changed, int *iters, int count, int round,
+hipError_t launch_batch_update(const BatchGraphDev *graphs, int *active, int *changed,
+                               int *iters, int *class_active_next, int count, int round,
                                int last, hipStream_t s) {
-  batch_update_kernel<<<dim3((count + EW - 1) / EW), dim3(EW), 0, s>>>(active, changed, iters,
-                                                                        count, round, last);
+  batch_update_kernel<<<dim3((count + EW - 1) / EW), dim3(EW), 0, s>>>(
+      graphs, active, changed, iters, class_active_next, count, round, last);
   return hipGetLastError();
 }
 

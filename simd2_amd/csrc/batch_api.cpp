@@ -32,7 +32,8 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct Launch {
   int op, cfg;
-  int offset, count;  // slice of the item table
+  int items;  // work items (tiles) of the launch
+  int grid;   // workgroups: 8 x the longest per-XCD list
 };
 
 }  // namespace
@@ -50,6 +51,9 @@ struct simd2_batch {
   int *d_state = nullptr;  // active | changed | iters, count each
   BatchItem *d_items = nullptr;
   size_t items_cap = 0;
+  BatchLaunchDev *d_launch = nullptr;  // per launch: its per-XCD item lists
+  int *d_class = nullptr;              // [max_rounds + 1][launches]: active graphs per class
+  size_t launch_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -82,11 +86,12 @@ int tiles_of(const simd2_batch &b, int g, int cfg, int *tr, int *tc) {
   return *tr * *tc;
 }
 
-// Work items of one launch: graphs largest first, each graph's tiles kept on
-// one XCD (workgroup i runs on XCD i % 8), XCDs balanced by work.  Slots an
-// XCD has no work for hold g = -1 and exit at once.
-void build_items(const simd2_batch &b, const std::vector<int> &gs, int cfg,
-                 std::vector<BatchItem> &out) {
+// Work items of one launch as 8 per-XCD lists appended to `out`: graphs
+// largest first, all tiles of a graph in one list (workgroup i runs on XCD
+// i % 8 and takes item i / 8 of list i % 8, so a graph's panels stay in one
+// L2), lists balanced by work.
+BatchLaunchDev build_items(const simd2_batch &b, const std::vector<int> &gs, int cfg,
+                           std::vector<BatchItem> &out) {
   std::vector<int> order(gs);
   std::stable_sort(order.begin(), order.end(),
                    [&](int x, int y) { return b.graphs[x].n > b.graphs[y].n; });
@@ -100,18 +105,22 @@ void build_items(const simd2_batch &b, const std::vector<int> &gs, int cfg,
       for (int c = 0; c < tc; ++c) lists[x].push_back({g, (r << 16) | c});
     load[x] += nt * tile_cost(b, g, cfg);
   }
-  size_t len = 0;
-  for (auto &l : lists) len = std::max(len, l.size());
-  for (size_t j = 0; j < len; ++j)
-    for (int x = 0; x < NUM_XCD; ++x)
-      out.push_back(j < lists[x].size() ? lists[x][j] : BatchItem{-1, 0});
+  BatchLaunchDev L;
+  for (int x = 0; x < NUM_XCD; ++x) {
+    L.off[x] = (int)out.size();
+    L.len[x] = (int)lists[x].size();
+    out.insert(out.end(), lists[x].begin(), lists[x].end());
+  }
+  return L;
 }
 
-// Groups graphs by (op, cfg) into launches (largest total work first) and
-// fills `items`.  FP32 graphs with cfg < 0 are left out (the autotuner plans
-// one size bin at a time), or_and graphs unless with_orand.
+// Groups graphs by (op, cfg) into launches (largest total work first), fills
+// `items` and the launch descriptors, and records each graph's launch in
+// cls[g].  FP32 graphs with cfg < 0 are left out (the autotuner plans one size
+// bin at a time), or_and graphs unless with_orand.
 void plan(const simd2_batch &b, const std::vector<int> &cfg, bool with_orand,
-          std::vector<Launch> &launches, std::vector<BatchItem> &items) {
+          std::vector<Launch> &launches, std::vector<BatchLaunchDev> &ldev,
+          std::vector<BatchItem> &items, std::vector<int> &cls) {
   std::map<std::pair<int, int>, std::vector<int>> classes;
   for (int g = 0; g < b.count; ++g) {
     if (b.graphs[g].n == 0) continue;
@@ -129,40 +138,78 @@ void plan(const simd2_batch &b, const std::vector<int> &cfg, bool with_orand,
   }
   std::sort(order.begin(), order.end());
   launches.clear();
+  ldev.clear();
   items.clear();
+  cls.assign(b.count, 0);
   for (auto &o : order) {
     Launch L;
     L.op = o.second.first;
     L.cfg = o.second.second;
-    L.offset = (int)items.size();
-    build_items(b, classes[o.second], L.cfg, items);
-    L.count = (int)items.size() - L.offset;
+    const int first = (int)items.size();
+    for (int g : classes[o.second]) cls[g] = (int)launches.size();
+    const BatchLaunchDev d = build_items(b, classes[o.second], L.cfg, items);
+    ldev.push_back(d);
+    L.items = (int)items.size() - first;
+    L.grid = NUM_XCD * *std::max_element(d.len, d.len + NUM_XCD);
     launches.push_back(L);
   }
 }
 
-int upload_items(simd2_batch &b, const std::vector<BatchItem> &items) {
+// Copies a plan to the device (growing the buffers when needed) and makes it
+// the one launch_round runs.
+int upload_plan(simd2_batch &b, const std::vector<Launch> &launches,
+                const std::vector<BatchLaunchDev> &ldev, const std::vector<BatchItem> &items,
+                const std::vector<int> &cls) {
+  hipError_t e = hipSuccess;
   if (items.size() > b.items_cap) {
     if (b.d_items) (void)hipFree(b.d_items);
     b.d_items = nullptr;
     b.items_cap = 0;
-    hipError_t e = hipMalloc(&b.d_items, items.size() * sizeof(BatchItem));
+    e = hipMalloc(&b.d_items, items.size() * sizeof(BatchItem));
     if (e != hipSuccess) return hip_status(e);
     b.items_cap = items.size();
   }
-  if (items.empty()) return SIMD2_SUCCESS;
-  return hip_status(hipMemcpy(b.d_items, items.data(), items.size() * sizeof(BatchItem),
-                              hipMemcpyHostToDevice));
+  if (launches.size() > b.launch_cap) {
+    if (b.d_launch) (void)hipFree(b.d_launch);
+    if (b.d_class) (void)hipFree(b.d_class);
+    b.d_launch = nullptr;
+    b.d_class = nullptr;
+    b.launch_cap = 0;
+    const size_t nl = launches.size();
+    e = hipMalloc(&b.d_launch, nl * sizeof(BatchLaunchDev));
+    if (e == hipSuccess) e = hipMalloc(&b.d_class, (b.max_rounds + 1) * nl * sizeof(int));
+    if (e != hipSuccess) return hip_status(e);
+    b.launch_cap = nl;
+  }
+  if (!items.empty())
+    e = hipMemcpy(b.d_items, items.data(), items.size() * sizeof(BatchItem),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess && !ldev.empty())
+    e = hipMemcpy(b.d_launch, ldev.data(), ldev.size() * sizeof(BatchLaunchDev),
+                  hipMemcpyHostToDevice);
+  for (int g = 0; g < b.count; ++g) b.desc[g].cls = cls[g];
+  if (e == hipSuccess && b.count > 0)
+    e = hipMemcpy(b.d_desc, b.desc.data(), b.desc.size() * sizeof(BatchGraphDev),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_status(e);
+  b.launches = launches;
+  return SIMD2_SUCCESS;
 }
 
-// Launches of one round (r) of the current plan on stream s.
+// Round r of the current plan on stream s: clear the next round's class
+// counts, then one launch per class (classes with no active graph left
+// return at once).
 hipError_t launch_round(simd2_batch &b, int r, hipStream_t s) {
-  for (const Launch &L : b.launches) {
-    hipError_t e = launch_batch_round(L.op, L.cfg, b.d_items + L.offset, L.count, b.d_desc,
-                                      b.pool, b.active(), b.changed(), r & 1, s);
-    if (e != hipSuccess) return e;
+  const int nl = (int)b.launches.size();
+  if (nl == 0) return hipSuccess;
+  hipError_t e = launch_fill_i32(b.d_class + (int64_t)(r + 1) * nl, 0, nl, s);
+  for (int i = 0; i < nl && e == hipSuccess; ++i) {
+    const Launch &L = b.launches[i];
+    e = launch_batch_round(L.op, L.cfg, L.grid, b.d_items, b.d_launch + i, b.d_desc, b.pool,
+                           b.active(), b.changed(), r & 1,
+                           r > 0 ? b.d_class + (int64_t)r * nl + i : nullptr, s);
   }
-  return hipSuccess;
+  return e;
 }
 
 void destroy_graph(simd2_batch &b) {
@@ -176,13 +223,14 @@ void destroy_graph(simd2_batch &b) {
 // final copy-back as one graph.
 int install(simd2_batch &b, const std::vector<int> &cfg) {
   std::vector<Launch> launches;
+  std::vector<BatchLaunchDev> ldev;
   std::vector<BatchItem> items;
-  plan(b, cfg, true, launches, items);
-  int st = upload_items(b, items);
+  std::vector<int> cls;
+  destroy_graph(b);  // the old graph refers to buffers the upload may replace
+  plan(b, cfg, true, launches, ldev, items, cls);
+  int st = upload_plan(b, launches, ldev, items, cls);
   if (st != SIMD2_SUCCESS) return st;
   b.cfg = cfg;
-  b.launches = launches;
-  destroy_graph(b);
   if (b.count == 0) return SIMD2_SUCCESS;
   hipStream_t cs;
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
@@ -193,7 +241,8 @@ int install(simd2_batch &b, const std::vector<int> &cfg) {
     for (int r = 0; r < b.max_rounds && le == hipSuccess; ++r) {
       le = launch_round(b, r, cs);
       if (le == hipSuccess)
-        le = launch_batch_update(b.active(), b.changed(), b.iters(), b.count, r,
+        le = launch_batch_update(b.d_desc, b.active(), b.changed(), b.iters(),
+                                 b.d_class + (int64_t)(r + 1) * b.launches.size(), b.count, r,
                                  r == b.max_rounds - 1, cs);
     }
     if (le == hipSuccess) le = launch_batch_finalize(b.d_desc, b.pool, b.iters(), b.count, cs);
@@ -318,6 +367,8 @@ int simd2_batch_destroy(simd2_batch *b) {
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_state) (void)hipFree(b->d_state);
   if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_launch) (void)hipFree(b->d_launch);
+  if (b->d_class) (void)hipFree(b->d_class);
   delete b;
   return SIMD2_SUCCESS;
 }
@@ -404,18 +455,19 @@ int simd2_batch_autotune(simd2_batch *b, hipStream_t stream) {
     return SIMD2_ERROR_INTERNAL;
   }
   const std::vector<int> saved_cfg = b->cfg;
-  const std::vector<Launch> saved_launches = b->launches;
   int st = SIMD2_SUCCESS;
+  destroy_graph(*b);  // trial plans replace the device tables it refers to
 
   // Times `reps` runs of round 0 of the plan for cfg (all graphs active).
   auto time_plan = [&](const std::vector<int> &cfg, bool with_orand, int reps,
                        float *ms) -> int {
     std::vector<Launch> launches;
+    std::vector<BatchLaunchDev> ldev;
     std::vector<BatchItem> items;
-    plan(*b, cfg, with_orand, launches, items);
-    int s = upload_items(*b, items);
+    std::vector<int> cls;
+    plan(*b, cfg, with_orand, launches, ldev, items, cls);
+    int s = upload_plan(*b, launches, ldev, items, cls);
     if (s != SIMD2_SUCCESS) return s;
-    b->launches = launches;
     hipError_t e = launch_batch_reset(b->d_desc, b->active(), b->changed(), b->iters(),
                                       b->count, stream);
     if (e == hipSuccess) e = launch_round(*b, 0, stream);  // warm-up
@@ -479,8 +531,6 @@ int simd2_batch_autotune(simd2_batch *b, hipStream_t stream) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st != SIMD2_SUCCESS) {
-    b->cfg = saved_cfg;
-    b->launches = saved_launches;
     (void)install(*b, saved_cfg);
     return st;
   }

@@ -83,20 +83,35 @@ struct BatchGraphDev {
   int kpad;        // K loop bound of the VALU kernel (multiple of 16)
   int epi;         // 1: result (+)= current matrix
   int mode;        // CheckMode of the change test
+  int cls;         // launch (op, tile configuration class) the graph belongs to
+  int pad_;
 };
 struct BatchItem {
-  int g;     // graph index, -1 for an idle slot
+  int g;     // graph index
   int tile;  // tile row << 16 | tile column (or word-block column)
 };
+// One launch: its items as 8 per-XCD lists (items[off[x] .. off[x] +
+// len[x])); workgroup b runs item b / 8 of list b % 8.
+struct BatchLaunchDev {
+  int off[8];
+  int len[8];
+};
+constexpr int BATCH_XCDS = 8;
 bool batch_op_supported(int op);
 void batch_pad_values(int op, float *pad_x, float *pad_y);
 void batch_tile_shape(int op, int cfg, int *bm, int *bn);
-hipError_t launch_batch_round(int op, int cfg, const BatchItem *items, int n_items,
-                              const BatchGraphDev *graphs, void *pool, const int *active,
-                              int *changed, int parity, hipStream_t s);
+// L: the launch's descriptor in device memory; grid: 8 x the longest list;
+// class_active: graphs of this launch's class still active (null: all are).
+hipError_t launch_batch_round(int op, int cfg, int grid, const BatchItem *items,
+                              const BatchLaunchDev *L, const BatchGraphDev *graphs, void *pool,
+                              const int *active, int *changed, int parity,
+                              const int *class_active, hipStream_t s);
 hipError_t launch_batch_reset(const BatchGraphDev *graphs, int *active, int *changed, int *iters,
                               int count, hipStream_t s);
-hipError_t launch_batch_update(int *active, int *changed, int *iters, int count, int round,
+// End of round: retire converged graphs; count the still-active graphs of
+// each class into class_active_next[cls].
+hipError_t launch_batch_update(const BatchGraphDev *graphs, int *active, int *changed,
+                               int *iters, int *class_active_next, int count, int round,
                                int last, hipStream_t s);
 hipError_t launch_batch_finalize(const BatchGraphDev *graphs, void *pool, const int *iters,
                                  int count, hipStream_t s);
@@ -134,6 +149,21 @@ hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, i
 hipError_t launch_fill_f32(float *p, float v, int64_t n, hipStream_t s);
 hipError_t launch_permute_copy(float *dst, int64_t ldd, const float *src, int64_t lds, int n,
                                const int *perm, int inverse, hipStream_t s);
+
+// Learned schedule (learned.hip).
+size_t graph_features_workspace(int n);
+hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int epi, float *f,
+                                 void *ws, hipStream_t s);
+hipError_t launch_graph_levels(int n, const float *X, int64_t ld, int *level, void *ws,
+                               hipStream_t s);
+int mlp_param_count();
+size_t mlp_workspace(int B);
+hipError_t launch_mlp_forward(const float *P, const float *x, int B, float *y, const float *norm,
+                              hipStream_t s);
+hipError_t launch_mlp_gradient(const float *P, const float *x, const float *t, const float *w,
+                               int B, float *grad, float *loss, void *ws, hipStream_t s);
+hipError_t launch_mlp_adam(float *P, float *m, float *v, const float *g, float lr, float c1,
+                           float c2, hipStream_t s);
 
 // Element-wise helpers (converge.hip).
 hipError_t launch_comp_update_f32(float *prev, float *cur, int *flag,

@@ -1,0 +1,528 @@
+// The learned schedule's device side (gfx950): graph features, breadth-first
+// searches, and a 24 -> 64 -> 64 -> 2 MLP with forward, backward and Adam
+// kernels.  All reductions run in a fixed order, so every result is
+// deterministic.
+#include <float.h>
+#include <math.h>
+
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace simd2 {
+
+namespace {
+
+constexpr int NF = 24, NH = 64, NO = 2;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NF, OFF_W2 = OFF_B1 + NH,
+              OFF_B2 = OFF_W2 + NH * NH, OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NO * NH,
+              NPARAM = OFF_B3 + NO;
+constexpr int GRAD_CHUNK = 16;  // samples per gradient workgroup
+constexpr int RS = 8;           // row statistics per vertex
+constexpr int BFS_T = 256;      // threads of a breadth-first search
+constexpr int BFS_W = 4;        // adjacency words per thread (n <= 32768)
+constexpr int NSRC = 8;         // sampled search sources
+
+// An entry is an edge when it is off the diagonal and a finite weight (not
+// the (+)-identity, nor the FLT_MAX "no edge" of the reference apps).
+__device__ __forceinline__ bool is_edge(float v, int i, int j) {
+  return i != j && fabsf(v) < FLT_MAX;
+}
+
+// Per row i: out-degree, sum |w|, sum w^2, min |w|, max |w|, sum |i-j|,
+// max |i-j|, edges with |i-j| <= 16 (doubles, fixed-order tree reduction).
+__global__ __launch_bounds__(256) void row_stats_kernel(const float *X, int64_t ld, int n,
+                                                        double *stats) {
+  __shared__ double sh[RS][256];
+  const int i = bid_x(), t = tid_x();
+  double deg = 0, ws = 0, wq = 0, wmin = 1e300, wmax = 0, ds = 0, dmax = 0, near = 0;
+  for (int j = t; j < n; j += 256) {
+    const float v = X[(int64_t)i * ld + j];
+    if (is_edge(v, i, j)) {
+      const double w = v, aw = fabs(w), d = abs(i - j);
+      deg += 1;
+      ws += aw;
+      wq += w * w;
+      wmin = fmin(wmin, aw);
+      wmax = fmax(wmax, aw);
+      ds += d;
+      dmax = fmax(dmax, d);
+      near += d <= 16 ? 1 : 0;
+    }
+  }
+  sh[0][t] = deg; sh[1][t] = ws; sh[2][t] = wq; sh[3][t] = wmin;
+  sh[4][t] = wmax; sh[5][t] = ds; sh[6][t] = dmax; sh[7][t] = near;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      for (int k = 0; k < RS; ++k) {
+        const double a = sh[k][t], b = sh[k][t + s];
+        sh[k][t] = k == 3 ? fmin(a, b) : (k == 4 || k == 6) ? fmax(a, b) : a + b;
+      }
+    }
+    __syncthreads();
+  }
+  if (t < RS) stats[(int64_t)i * RS + t] = sh[t][0];
+}
+
+// In-degrees: workgroup (column block, row chunk) counts its edges.
+__global__ __launch_bounds__(256) void in_degree_kernel(const float *X, int64_t ld, int n,
+                                                        int rows_per_chunk, int *indeg) {
+  const int j = bid_x() * 256 + tid_x();
+  const int r0 = bid_y() * rows_per_chunk;
+  if (j >= n) return;
+  int c = 0;
+  const int r1 = min(n, r0 + rows_per_chunk);
+  for (int i = r0; i < r1; ++i) c += is_edge(X[(int64_t)i * ld + j], i, j) ? 1 : 0;
+  if (c) atomicAdd(indeg + j, c);
+}
+
+// Reverse-edge count over the sampled rows i = k * step: out[0] += edges
+// (i, j), out[1] += those with (j, i) an edge too.
+__global__ __launch_bounds__(256) void symmetry_kernel(const float *X, int64_t ld, int n,
+                                                       int step, int *out) {
+  const int i = bid_x() * step;
+  if (i >= n) return;
+  int e = 0, s = 0;
+  for (int j = tid_x(); j < n; j += 256) {
+    if (is_edge(X[(int64_t)i * ld + j], i, j)) {
+      ++e;
+      s += is_edge(X[(int64_t)j * ld + i], j, i) ? 1 : 0;
+    }
+  }
+  if (e) atomicAdd(out, e);
+  if (s) atomicAdd(out + 1, s);
+}
+
+// Bit-packed adjacency: bit (j & 31) of word adj[i][j / 32] is edge (i, j).
+// A wave turns 64 entries of a row into two words with one ballot.
+__global__ __launch_bounds__(256) void pack_adj_kernel(const float *X, int64_t ld, int n, int nw,
+                                                       uint32_t *adj) {
+  const int i = bid_y();
+  const int j = bid_x() * 256 + tid_x();
+  const bool e = j < n && is_edge(X[(int64_t)i * ld + j], i, j);
+  const uint64_t m = __ballot(e);
+  const int lane = tid_x() % 64;
+  const int w0 = (bid_x() * 256 + (tid_x() - lane)) / 32;
+  if (lane == 0 && w0 < nw) adj[(int64_t)i * nw + w0] = (uint32_t)m;
+  if (lane == 32 && w0 + 1 < nw) adj[(int64_t)i * nw + w0 + 1] = (uint32_t)(m >> 32);
+}
+
+// Breadth-first search from src over the packed adjacency, by one
+// workgroup.  The frontier is a vertex list in global memory (cur / nxt, n
+// ints each); thread t owns adjacency words t + 256 q.  Writes depth (the
+// last level reached) and reached (vertex count) to res[0..1] and, when
+// level is set, each vertex's level (-1: unreached).  Every level adds at
+// least one vertex or ends the search, so it stops within n levels.
+__device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *nxt, int *level,
+                    int *res, uint32_t *visited, int *s_cnt) {
+  const int t = tid_x();
+  for (int w = t; w < nw; w += BFS_T) visited[w] = 0;
+  if (level)
+    for (int v = t; v < n; v += BFS_T) level[v] = v == src ? 0 : -1;
+  __syncthreads();
+  if (t == 0) {
+    visited[src / 32] |= 1u << (src % 32);
+    cur[0] = src;
+    s_cnt[0] = 1;  // frontier size
+    s_cnt[1] = 0;  // next frontier size
+  }
+  __syncthreads();
+  int depth = 0, reached = 1;
+  for (;;) {
+    const int cnt = s_cnt[0];
+    uint32_t acc[BFS_W] = {0u, 0u, 0u, 0u};
+    for (int base = 0; base < cnt; base += 8) {
+      int u[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) u[k] = base + k < cnt ? cur[base + k] : -1;
+#pragma unroll
+      for (int q = 0; q < BFS_W; ++q) {
+        const int w = t + BFS_T * q;
+        if (w < nw) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (u[k] >= 0) acc[q] |= adj[(int64_t)u[k] * nw + w];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < BFS_W; ++q) {
+      const int w = t + BFS_T * q;
+      if (w < nw) {
+        uint32_t fresh = acc[q] & ~visited[w];
+        if (fresh) {
+          visited[w] |= fresh;
+          int at = atomicAdd(&s_cnt[1], __popc(fresh));
+          while (fresh) {
+            const int b = __ffs(fresh) - 1;
+            fresh &= fresh - 1;
+            const int v = w * 32 + b;
+            nxt[at++] = v;
+            if (level) level[v] = depth + 1;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const int ncnt = s_cnt[1];
+    __syncthreads();
+    if (ncnt == 0) break;
+    ++depth;
+    reached += ncnt;
+    int *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+    if (t == 0) {
+      s_cnt[0] = ncnt;
+      s_cnt[1] = 0;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    res[0] = depth;
+    res[1] = reached;
+  }
+  __syncthreads();
+}
+
+// Feature searches: workgroup k searches from vertex k * n / 8.
+__global__ __launch_bounds__(BFS_T) void bfs_sources_kernel(const uint32_t *adj, int n, int nw,
+                                                            int *lists, int *res) {
+  __shared__ uint32_t visited[BFS_T * BFS_W];
+  __shared__ int s_cnt[2];
+  const int k = bid_x();
+  const int src = (int)((int64_t)k * n / NSRC);
+  int *cur = lists + (int64_t)k * 2 * n;
+  bfs(adj, n, nw, src, cur, cur + n, nullptr, res + 2 * k, visited, s_cnt);
+}
+
+// Vertex levels for the order: search from vertex 0, restart from the
+// lowest-numbered vertex of its last level (a pseudo-peripheral vertex) and
+// keep the levels of the second search.
+__global__ __launch_bounds__(BFS_T) void bfs_levels_kernel(const uint32_t *adj, int n, int nw,
+                                                           int *lists, int *level, int *res) {
+  __shared__ uint32_t visited[BFS_T * BFS_W];
+  __shared__ int s_cnt[2];
+  __shared__ int s_far;
+  bfs(adj, n, nw, 0, lists, lists + n, level, res, visited, s_cnt);
+  const int depth = res[0];
+  if (tid_x() == 0) s_far = n;
+  __syncthreads();
+  int far = n;
+  for (int v = tid_x(); v < n; v += BFS_T)
+    if (level[v] == depth) far = min(far, v);
+  if (far < n) atomicMin(&s_far, far);
+  __syncthreads();
+  bfs(adj, n, nw, s_far, lists, lists + n, level, res + 2, visited, s_cnt);
+}
+
+// The 24 features (one workgroup; fixed-order reductions in double).
+constexpr int NRED = 13;
+__global__ __launch_bounds__(256) void features_kernel(const double *rstats, const int *indeg,
+                                                       const int *sym, const int *bfs_res, int n,
+                                                       int max_based, int epi, float *f) {
+  __shared__ double sh[NRED][256];
+  const int t = tid_x();
+  double v[NRED] = {0, 0, 0, 0, 0, 0, 1e300, 0, 0, 0, 0, 0, 0};
+  for (int i = t; i < n; i += 256) {
+    const double *r = rstats + (int64_t)i * RS;
+    const double din = indeg[i];
+    v[0] += r[0];                      // edges
+    v[1] += r[0] * r[0];               // sum out-degree^2
+    v[2] = fmax(v[2], r[0]);           // max out-degree
+    v[3] += r[0] == 0 ? 1 : 0;         // vertices without out-edges
+    v[4] += r[1];                      // sum |w|
+    v[5] += r[2];                      // sum w^2
+    if (r[0] > 0) v[6] = fmin(v[6], r[3]);  // min |w|
+    v[7] = fmax(v[7], r[4]);           // max |w|
+    v[8] += r[5];                      // sum |i - j|
+    v[9] = fmax(v[9], r[6]);           // max |i - j|
+    v[10] += r[7];                     // edges with |i - j| <= 16
+    v[11] += din == 0 ? 1 : 0;         // vertices without in-edges
+    v[12] += din * din;                // sum in-degree^2
+  }
+  for (int k = 0; k < NRED; ++k) sh[k][t] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < NRED; ++k) {
+        const double a = sh[k][t], b = sh[k][t + s];
+        sh[k][t] = (k == 2 || k == 7 || k == 9) ? fmax(a, b) : k == 6 ? fmin(a, b) : a + b;
+      }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const double N = n, m = sh[0][0];
+  const double mean = m / N;
+  const double sd_out = sqrt(fmax(sh[1][0] / N - mean * mean, 0.0));
+  const double sd_in = sqrt(fmax(sh[12][0] / N - mean * mean, 0.0));
+  const double density = n > 1 ? m / (N * (N - 1)) : 0.0;
+  const double wmax = sh[7][0];
+  const double wmean = m > 0 ? sh[4][0] / m : 0.0;
+  const double wsd = m > 0 ? sqrt(fmax(sh[5][0] / m - wmean * wmean, 0.0)) : 0.0;
+  double dmean = 0, dmx = 0, reach = 0, width = 0;
+  for (int k = 0; k < NSRC; ++k) {
+    const double d = bfs_res[2 * k], r = bfs_res[2 * k + 1];
+    dmean += d / NSRC;
+    dmx = fmax(dmx, d);
+    reach += r / N / NSRC;
+    width += r / (d + 1) / N / NSRC;
+  }
+  double out[NF];
+  out[0] = log2(N) / 16;
+  out[1] = density;
+  out[2] = (log10(density + 1e-9) + 9) / 9;
+  out[3] = log2(1 + mean) / 16;
+  out[4] = sd_out / (1 + mean);
+  out[5] = log2(1 + sh[2][0]) / 16;
+  out[6] = sh[3][0] / N;
+  out[7] = sh[11][0] / N;
+  out[8] = sd_in / (1 + mean);
+  out[9] = wmax > 0 ? wmean / wmax : 0;
+  out[10] = m > 0 ? fmin(wsd / (wmean + 1e-12), 10.0) : 0;
+  out[11] = (m > 0 && wmax > 0) ? sh[6][0] / wmax : 0;
+  out[12] = log10(1 + wmax) / 10;
+  out[13] = sym[0] > 0 ? (double)sym[1] / sym[0] : 0;
+  out[14] = m > 0 ? sh[8][0] / m / N : 0;
+  out[15] = m > 0 ? sh[9][0] / N : 0;
+  out[16] = m > 0 ? sh[10][0] / m : 0;
+  out[17] = log2(1 + dmean) / 16;
+  out[18] = log2(1 + dmx) / 16;
+  out[19] = reach;
+  out[20] = width;
+  out[21] = max_based ? 1 : 0;
+  out[22] = epi ? 1 : 0;
+  out[23] = (ceil(log2(fmax(dmx, 1.0))) + 1) / 16;
+  for (int k = 0; k < NF; ++k) f[k] = (float)out[k];
+}
+
+// ---- MLP -------------------------------------------------------------------
+
+// y = MLP(x) for one sample per 64-thread workgroup.  norm (optional):
+// feature mean[24], scale[24], output scale[2], offset[2].
+__global__ __launch_bounds__(64) void mlp_forward_kernel(const float *P, const float *x, int B,
+                                                         float *y, const float *norm) {
+  __shared__ float xs[NF], h1[NH], h2[NH];
+  const int b = bid_x(), j = tid_x();
+  if (b >= B) return;
+  if (j < NF) {
+    const float v = x[(int64_t)b * NF + j];
+    xs[j] = norm ? (v - norm[j]) * norm[NF + j] : v;
+  }
+  __syncthreads();
+  float a = P[OFF_B1 + j];
+  for (int f = 0; f < NF; ++f) a += P[OFF_W1 + j * NF + f] * xs[f];
+  h1[j] = a > 0.f ? a : 0.f;
+  __syncthreads();
+  a = P[OFF_B2 + j];
+  for (int k = 0; k < NH; ++k) a += P[OFF_W2 + j * NH + k] * h1[k];
+  h2[j] = a > 0.f ? a : 0.f;
+  __syncthreads();
+  if (j < NO) {
+    float o = P[OFF_B3 + j];
+    for (int k = 0; k < NH; ++k) o += P[OFF_W3 + j * NH + k] * h2[k];
+    y[(int64_t)b * NO + j] = norm ? o * norm[2 * NF + j] + norm[2 * NF + NO + j] : o;
+  }
+}
+
+// Gradient of L = (1/B) sum_b sum_o w_o (y_bo - t_bo)^2 over GRAD_CHUNK
+// samples per 64-thread workgroup: thread j accumulates the gradient of
+// hidden unit j's parameters (W1 row j, b1[j], W2 row j, b2[j], W3[.][j],
+// and b3[j] for j < 2) in registers, sample after sample, and writes the
+// chunk's partial gradient (and loss, last entry) to partial[chunk].
+__global__ __launch_bounds__(64) void mlp_grad_kernel(const float *P, const float *x,
+                                                      const float *t, const float *w, int B,
+                                                      float *partial) {
+  __shared__ float xs[NF], h1[NH], h2[NH], dh2[NH], dh1[NH], dy[NO], df[NO];
+  const int c = bid_x(), j = tid_x();
+  float g1[NF], g2[NH], g3[NO], gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, loss = 0.f;
+  _Pragma("unroll") for (int f = 0; f < NF; ++f) g1[f] = 0.f;
+  _Pragma("unroll") for (int k = 0; k < NH; ++k) g2[k] = 0.f;
+  g3[0] = g3[1] = 0.f;
+  const float invB = 1.0f / (float)B;
+  const int s1 = min(B, (c + 1) * GRAD_CHUNK);
+  for (int s = c * GRAD_CHUNK; s < s1; ++s) {
+    if (j < NF) xs[j] = x[(int64_t)s * NF + j];
+    __syncthreads();
+    float a = P[OFF_B1 + j];
+    for (int f = 0; f < NF; ++f) a += P[OFF_W1 + j * NF + f] * xs[f];
+    h1[j] = a > 0.f ? a : 0.f;
+    __syncthreads();
+    a = P[OFF_B2 + j];
+    for (int k = 0; k < NH; ++k) a += P[OFF_W2 + j * NH + k] * h1[k];
+    h2[j] = a > 0.f ? a : 0.f;
+    __syncthreads();
+    if (j < NO) {
+      float o = P[OFF_B3 + j];
+      for (int k = 0; k < NH; ++k) o += P[OFF_W3 + j * NH + k] * h2[k];
+      const float d = o - t[(int64_t)s * NO + j];
+      df[j] = d;
+      dy[j] = 2.f * w[j] * d * invB;
+    }
+    __syncthreads();
+    if (j == 0) loss += (w[0] * df[0] * df[0] + w[1] * df[1] * df[1]) * invB;
+    const float h2j = h2[j];
+    g3[0] += dy[0] * h2j;
+    g3[1] += dy[1] * h2j;
+    if (j < NO) gb3 += dy[j];
+    const float back2 = dy[0] * P[OFF_W3 + j] + dy[1] * P[OFF_W3 + NH + j];
+    const float d2 = h2j > 0.f ? back2 : 0.f;
+    dh2[j] = d2;
+    __syncthreads();
+    float back1 = 0.f;
+    for (int jj = 0; jj < NH; ++jj) back1 += dh2[jj] * P[OFF_W2 + jj * NH + j];
+    dh1[j] = h1[j] > 0.f ? back1 : 0.f;
+    _Pragma("unroll") for (int k = 0; k < NH; ++k) g2[k] += d2 * h1[k];
+    gb2 += d2;
+    __syncthreads();
+    const float d1 = dh1[j];
+    _Pragma("unroll") for (int f = 0; f < NF; ++f) g1[f] += d1 * xs[f];
+    gb1 += d1;
+    __syncthreads();
+  }
+  float *out = partial + (int64_t)c * (NPARAM + 1);
+  _Pragma("unroll") for (int f = 0; f < NF; ++f) out[OFF_W1 + j * NF + f] = g1[f];
+  out[OFF_B1 + j] = gb1;
+  _Pragma("unroll") for (int k = 0; k < NH; ++k) out[OFF_W2 + j * NH + k] = g2[k];
+  out[OFF_B2 + j] = gb2;
+  out[OFF_W3 + j] = g3[0];
+  out[OFF_W3 + NH + j] = g3[1];
+  if (j < NO) out[OFF_B3 + j] = gb3;
+  if (j == 0) out[NPARAM] = loss;
+}
+
+// grad[p] = sum over chunks, in chunk order (p == NPARAM: the loss).
+__global__ __launch_bounds__(256) void mlp_reduce_kernel(const float *partial, int chunks,
+                                                         float *grad, float *loss) {
+  const int p = bid_x() * 256 + tid_x();
+  if (p > NPARAM) return;
+  float s = 0.f;
+  for (int c = 0; c < chunks; ++c) s += partial[(int64_t)c * (NPARAM + 1) + p];
+  if (p < NPARAM) {
+    if (grad) grad[p] = s;
+  } else if (loss) {
+    *loss = s;
+  }
+}
+
+// Adam with bias corrections c1 = 1 / (1 - beta1^t), c2 = 1 / (1 - beta2^t).
+__global__ __launch_bounds__(256) void adam_kernel(float *P, float *m, float *v, const float *g,
+                                                   float lr, float c1, float c2) {
+  const int p = bid_x() * 256 + tid_x();
+  if (p >= NPARAM) return;
+  const float gp = g[p];
+  const float mp = 0.9f * m[p] + 0.1f * gp;
+  const float vp = 0.999f * v[p] + 0.001f * gp * gp;
+  m[p] = mp;
+  v[p] = vp;
+  P[p] -= lr * (mp * c1) / (sqrtf(vp * c2) + 1e-8f);
+}
+
+struct FeatLayout {
+  size_t rstats, indeg, sym, adj, lists, res, level, total;
+};
+
+FeatLayout feat_layout(int n) {
+  FeatLayout L;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t at = off;
+    off += (b + 255) / 256 * 256;
+    return at;
+  };
+  const int nw = (n + 31) / 32;
+  L.rstats = take((size_t)n * RS * sizeof(double));
+  L.indeg = take((size_t)n * sizeof(int));
+  L.sym = take(2 * sizeof(int));
+  L.adj = take((size_t)n * nw * sizeof(uint32_t));
+  L.lists = take((size_t)NSRC * 2 * n * sizeof(int));
+  L.res = take(2 * NSRC * sizeof(int));
+  L.level = take((size_t)n * sizeof(int));
+  L.total = off;
+  return L;
+}
+
+hipError_t pack_adjacency(const float *X, int64_t ld, int n, uint32_t *adj, hipStream_t s) {
+  const int nw = (n + 31) / 32;
+  pack_adj_kernel<<<dim3((n + 255) / 256, n), dim3(256), 0, s>>>(X, ld, n, nw, adj);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t graph_features_workspace(int n) { return n > 0 ? feat_layout(n).total : 0; }
+
+hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int epi, float *f,
+                                 void *ws, hipStream_t s) {
+  const FeatLayout L = feat_layout(n);
+  char *w = static_cast<char *>(ws);
+  double *rstats = reinterpret_cast<double *>(w + L.rstats);
+  int *indeg = reinterpret_cast<int *>(w + L.indeg);
+  int *sym = reinterpret_cast<int *>(w + L.sym);
+  uint32_t *adj = reinterpret_cast<uint32_t *>(w + L.adj);
+  int *lists = reinterpret_cast<int *>(w + L.lists);
+  int *res = reinterpret_cast<int *>(w + L.res);
+  const int nw = (n + 31) / 32;
+  hipError_t e = launch_fill_i32(indeg, 0, n, s);
+  if (e == hipSuccess) e = launch_fill_i32(sym, 0, 2, s);
+  if (e != hipSuccess) return e;
+  row_stats_kernel<<<dim3(n), dim3(256), 0, s>>>(X, ld, n, rstats);
+  const int chunk = 512;
+  in_degree_kernel<<<dim3((n + 255) / 256, (n + chunk - 1) / chunk), dim3(256), 0, s>>>(
+      X, ld, n, chunk, indeg);
+  const int step = std::max(1, n / 256);
+  symmetry_kernel<<<dim3((n + step - 1) / step), dim3(256), 0, s>>>(X, ld, n, step, sym);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = pack_adjacency(X, ld, n, adj, s);
+  if (e != hipSuccess) return e;
+  bfs_sources_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, lists, res);
+  const int max_based = (op == 2 || op == 3 || op == 4) ? 1 : 0;
+  features_kernel<<<dim3(1), dim3(256), 0, s>>>(rstats, indeg, sym, res, n, max_based, epi, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_graph_levels(int n, const float *X, int64_t ld, int *level, void *ws,
+                               hipStream_t s) {
+  const FeatLayout L = feat_layout(n);
+  char *w = static_cast<char *>(ws);
+  uint32_t *adj = reinterpret_cast<uint32_t *>(w + L.adj);
+  int *lists = reinterpret_cast<int *>(w + L.lists);
+  int *res = reinterpret_cast<int *>(w + L.res);
+  hipError_t e = pack_adjacency(X, ld, n, adj, s);
+  if (e != hipSuccess) return e;
+  bfs_levels_kernel<<<dim3(1), dim3(BFS_T), 0, s>>>(adj, n, (n + 31) / 32, lists, level, res);
+  return hipGetLastError();
+}
+
+int mlp_param_count() { return NPARAM; }
+
+hipError_t launch_mlp_forward(const float *P, const float *x, int B, float *y, const float *norm,
+                              hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  mlp_forward_kernel<<<dim3(B), dim3(64), 0, s>>>(P, x, B, y, norm);
+  return hipGetLastError();
+}
+
+size_t mlp_workspace(int B) {
+  const size_t chunks = (size_t)(B + GRAD_CHUNK - 1) / GRAD_CHUNK;
+  return chunks * (NPARAM + 1) * sizeof(float) + 256 + (NPARAM + 1) * sizeof(float);
+}
+
+hipError_t launch_mlp_gradient(const float *P, const float *x, const float *t, const float *w,
+                               int B, float *grad, float *loss, void *ws, hipStream_t s) {
+  const int chunks = (B + GRAD_CHUNK - 1) / GRAD_CHUNK;
+  float *partial = static_cast<float *>(ws);
+  mlp_grad_kernel<<<dim3(chunks), dim3(64), 0, s>>>(P, x, t, w, B, partial);
+  mlp_reduce_kernel<<<dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, s>>>(partial, chunks, grad,
+                                                                        loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float *P, float *m, float *v, const float *g, float lr, float c1,
+                           float c2, hipStream_t s) {
+  adam_kernel<<<dim3((NPARAM + 255) / 256), dim3(256), 0, s>>>(P, m, v, g, lr, c1, c2);
+  return hipGetLastError();
+}
+
+}  // namespace simd2

@@ -1,0 +1,193 @@
+// Host driver of the scheduled converge loop (sched.hip): workspace layout,
+// the squaring sequence and the deferred flag reads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/simd2/converge.h"
+#include "../../include/simd2/schedule.h"
+#include "../../include/simd2/srgemm.h"
+#include "internal.hpp"
+
+using namespace simd2;
+
+namespace {
+
+int hip_status(hipError_t e) {
+  if (e == hipSuccess) return SIMD2_SUCCESS;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return SIMD2_ERROR_NO_DEVICE;
+  return SIMD2_ERROR_INTERNAL;
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+bool sched_op_supported(int op) {
+  return op == SIMD2_MIN_PLUS || op == SIMD2_MAX_PLUS || op == SIMD2_MAX_MIN ||
+         op == SIMD2_MIN_MAX;
+}
+
+// Workspace carve-up (every piece 256-byte aligned).
+struct Layout {
+  int P, kpad, tiles_r, tiles_c, slabs;
+  size_t buf[2], xocc[2], yocc[2], rowchg, colchg, conv, stats, total;
+};
+
+bool layout(int n, int cfg, int max_itr, Layout *L) {
+  if (n <= 0 || n > 16 * SCHED_MAX_SLABS || max_itr < 0) return false;
+  if (cfg < 0 || cfg >= CFG_COUNT) return false;
+  const ValuCfgInfo c = valu_cfg_info(cfg);
+  L->P = (int)round_up(n, 128);
+  L->kpad = (int)round_up(n, 16);
+  L->tiles_r = (n + c.bm - 1) / c.bm;
+  L->tiles_c = (n + c.bn - 1) / c.bn;
+  L->slabs = (n + 15) / 16;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (size_t)round_up((int64_t)bytes, 256);
+    return at;
+  };
+  const size_t mat = (size_t)L->P * L->P * sizeof(float);
+  L->buf[0] = take(mat);
+  L->buf[1] = take(mat);
+  for (int k = 0; k < 2; ++k) {
+    L->xocc[k] = take((size_t)L->tiles_r * L->slabs);
+    L->yocc[k] = take((size_t)L->tiles_c * L->slabs);
+  }
+  const size_t rounds = (size_t)max_itr + 1;
+  L->rowchg = take(rounds * L->tiles_r * sizeof(int));
+  L->colchg = take(rounds * L->tiles_c * sizeof(int));
+  L->conv = take(rounds * sizeof(int));
+  L->stats = take(2 * sizeof(unsigned long long));
+  L->total = off;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t simd2_converge_scheduled_workspace(int n, int tile_config, int max_itr) {
+  const int cfg = tile_config >= 0 ? tile_config : valu_pick_cfg(SIMD2_MIN_PLUS, n, n, n);
+  Layout L;
+  return layout(n, cfg, max_itr, &L) ? L.total : 0;
+}
+
+int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
+                             int check_mode, int max_itr, const simd2_schedule *schedule,
+                             const int *perm, void *workspace, size_t workspace_bytes,
+                             hipStream_t stream, int *num_itr, simd2_sched_report *report) {
+  if (num_itr) *num_itr = 0;
+  if (report) *report = simd2_sched_report{};
+  if (!sched_op_supported(op)) return SIMD2_ERROR_NOT_SUPPORTED;
+  if (check_mode < 0 || check_mode > 2) return SIMD2_ERROR_NOT_SUPPORTED;
+  if (n < 0 || max_itr < 0) return SIMD2_ERROR_INVALID_PROBLEM;
+  if (n == 0) return SIMD2_SUCCESS;
+  if (ldx < n || n > 16 * SCHED_MAX_SLABS) return SIMD2_ERROR_INVALID_PROBLEM;
+  if (!X || !workspace) return SIMD2_ERROR_NULL_POINTER;
+  simd2_schedule sc = schedule ? *schedule : simd2_schedule{-1, 0};
+  if (sc.tile_config >= CFG_COUNT || sc.predicted_itr < 0) return SIMD2_ERROR_INVALID_PROBLEM;
+  const int cfg = sc.tile_config >= 0 ? sc.tile_config : valu_pick_cfg(op, n, n, n);
+  Layout L;
+  if (!layout(n, cfg, max_itr, &L)) return SIMD2_ERROR_INVALID_PROBLEM;
+  if (workspace_bytes < L.total) return SIMD2_ERROR_WORKSPACE;
+
+  char *ws = static_cast<char *>(workspace);
+  float *buf[2] = {reinterpret_cast<float *>(ws + L.buf[0]),
+                   reinterpret_cast<float *>(ws + L.buf[1])};
+  uint8_t *xocc[2] = {reinterpret_cast<uint8_t *>(ws + L.xocc[0]),
+                      reinterpret_cast<uint8_t *>(ws + L.xocc[1])};
+  uint8_t *yocc[2] = {reinterpret_cast<uint8_t *>(ws + L.yocc[0]),
+                      reinterpret_cast<uint8_t *>(ws + L.yocc[1])};
+  int *rowchg = reinterpret_cast<int *>(ws + L.rowchg);
+  int *colchg = reinterpret_cast<int *>(ws + L.colchg);
+  int *conv = reinterpret_cast<int *>(ws + L.conv);
+  unsigned long long *stats = reinterpret_cast<unsigned long long *>(ws + L.stats);
+  const float inf = __builtin_inff();
+  const float ident = (op == SIMD2_MIN_PLUS || op == SIMD2_MIN_MAX) ? inf : -inf;
+
+  // Padded copies: everything outside the live n x n block is the identity.
+  const int64_t P2 = (int64_t)L.P * L.P;
+  hipError_t e = launch_fill_f32(buf[0], ident, P2, stream);
+  if (e == hipSuccess) e = launch_fill_f32(buf[1], ident, P2, stream);
+  if (e == hipSuccess) e = launch_permute_copy(buf[0], L.P, X, ldx, n, perm, 0, stream);
+  const int64_t flag_ints = (int64_t)(max_itr + 1) * (L.tiles_r + L.tiles_c + 1);
+  if (e == hipSuccess) e = launch_fill_i32(rowchg, 0, (int64_t)(max_itr + 1) * L.tiles_r, stream);
+  if (e == hipSuccess) e = launch_fill_i32(colchg, 0, (int64_t)(max_itr + 1) * L.tiles_c, stream);
+  if (e == hipSuccess) e = launch_fill_i32(conv, 0, max_itr + 1, stream);
+  if (e == hipSuccess) e = launch_fill_i32(reinterpret_cast<int *>(stats), 0, 4, stream);
+  (void)flag_ints;
+  if (e == hipSuccess)
+    e = launch_sched_occupancy(op, cfg, buf[0], L.P, n, L.tiles_r, L.tiles_c, L.slabs, xocc[0],
+                               yocc[0], stream);
+  if (e != hipSuccess) return hip_status(e);
+
+  int t = 0, T = -1, syncs = 0;
+  int chunk = sc.predicted_itr > 0 ? sc.predicted_itr : 1;
+  std::vector<int> flags;
+  while (t < max_itr && T < 0) {
+    const int m = std::min(chunk, max_itr - t);
+    for (int j = 0; j < m && e == hipSuccess; ++j) {
+      const int s = t + j;
+      SchedArgs a;
+      a.cur = buf[s & 1];
+      a.nxt = buf[(s + 1) & 1];
+      a.ld = L.P;
+      a.n = n;
+      a.kpad = L.kpad;
+      a.epi = do_epilogue ? 1 : 0;
+      a.mode = check_mode;
+      a.tiles_r = L.tiles_r;
+      a.tiles_c = L.tiles_c;
+      a.slabs = L.slabs;
+      a.xocc_in = xocc[s & 1];
+      a.xocc_out = xocc[(s + 1) & 1];
+      a.yocc_in = yocc[s & 1];
+      a.yocc_out = yocc[(s + 1) & 1];
+      a.rowchg_prev = s > 0 ? rowchg + (int64_t)(s - 1) * L.tiles_r : nullptr;
+      a.colchg_prev = s > 0 ? colchg + (int64_t)(s - 1) * L.tiles_c : nullptr;
+      a.rowchg = rowchg + (int64_t)s * L.tiles_r;
+      a.colchg = colchg + (int64_t)s * L.tiles_c;
+      a.conv_prev = s > 0 ? conv + s - 1 : nullptr;
+      a.conv = conv + s;
+      a.stats = stats;
+      e = launch_sched_square(op, cfg, a, stream);
+    }
+    if (e != hipSuccess) return hip_status(e);
+    flags.resize(m);
+    e = hipMemcpyAsync(flags.data(), conv + t, m * sizeof(int), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_status(e);
+    ++syncs;
+    for (int j = 0; j < m; ++j)
+      if (flags[j] == 0) {
+        T = t + j;
+        break;
+      }
+    if (report) report->squarings_enqueued += m;
+    t += m;
+    chunk = 1;
+  }
+  const int done = T >= 0 ? T + 1 : t;  // squarings whose result counts
+  float *result = buf[done & 1];
+  e = launch_permute_copy(X, ldx, result, L.P, n, perm, 1, stream);
+  unsigned long long st[2] = {0, 0};
+  if (e == hipSuccess && report)
+    e = hipMemcpyAsync(st, stats, sizeof(st), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_status(e);
+  if (num_itr) *num_itr = done;
+  if (report) {
+    const unsigned long long tiles = (unsigned long long)L.tiles_r * L.tiles_c;
+    report->host_syncs = syncs;
+    report->tiles_total = tiles * done;
+    report->tiles_computed = st[0];
+    report->slab_tiles_total = tiles * L.slabs * done;
+    report->slab_tiles_computed = st[1];
+  }
+  return T >= 0 ? SIMD2_SUCCESS : SIMD2_ERROR_NOT_CONVERGED;
+}
+
+}  // extern "C"

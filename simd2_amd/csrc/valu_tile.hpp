@@ -14,6 +14,11 @@ namespace {
 constexpr int GROUP = 8;
 constexpr int NUM_XCD = 8;
 
+// Native 4-float vector for the slab staging registers: the HIP float4
+// struct is copied with memcpy, which scalar replacement does not always
+// undo, and the staged slab then lives in scratch.
+typedef float vf4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ void map_tile(int b, int tiles_r, int tiles_c,
                                          int &tr, int &tc) {
   const int nt = tiles_r * tiles_c;
@@ -43,8 +48,8 @@ __device__ __forceinline__ bool differs(float o, float p, int mode) {
 // registers.  Fast path: 128-bit loads; checked path: per-element bounds
 // with the op's padding values.
 template <class Op, int BM, int BN, int BK, int NT, bool CHECKED>
-__device__ __forceinline__ void load_slab(float4 (&xv)[BM * BK / 4 / NT],
-                                          float4 (&yv)[BK * BN / 4 / NT],
+__device__ __forceinline__ void load_slab(vf4 (&xv)[BM * BK / 4 / NT],
+                                          vf4 (&yv)[BK * BN / 4 / NT],
                                           const float *__restrict__ X, int64_t ldx,
                                           const float *__restrict__ Y, int64_t ldy,
                                           int R, int Cn, int K, int row0, int col0,
@@ -56,13 +61,13 @@ __device__ __forceinline__ void load_slab(float4 (&xv)[BM * BK / 4 / NT],
     const int r = row0 + f / (BK / 4);
     const int k = k0 + (f % (BK / 4)) * 4;
     if constexpr (!CHECKED) {
-      xv[h] = *reinterpret_cast<const float4 *>(X + (int64_t)r * ldx + k);
+      xv[h] = *reinterpret_cast<const vf4 *>(X + (int64_t)r * ldx + k);
     } else {
       float t[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         t[q] = (r < R && k + q < K) ? X[(int64_t)r * ldx + k + q] : Op::pad_x;
-      xv[h] = make_float4(t[0], t[1], t[2], t[3]);
+      xv[h] = vf4{t[0], t[1], t[2], t[3]};
     }
   }
 #pragma unroll
@@ -71,21 +76,21 @@ __device__ __forceinline__ void load_slab(float4 (&xv)[BM * BK / 4 / NT],
     const int k = k0 + f / (BN / 4);
     const int c = col0 + (f % (BN / 4)) * 4;
     if constexpr (!CHECKED) {
-      yv[h] = *reinterpret_cast<const float4 *>(Y + (int64_t)k * ldy + c);
+      yv[h] = *reinterpret_cast<const vf4 *>(Y + (int64_t)k * ldy + c);
     } else {
       float t[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         t[q] = (k < K && c + q < Cn) ? Y[(int64_t)k * ldy + c + q] : Op::pad_y;
-      yv[h] = make_float4(t[0], t[1], t[2], t[3]);
+      yv[h] = vf4{t[0], t[1], t[2], t[3]};
     }
   }
 }
 
 template <int BM, int BN, int BK, int NT, int XLD>
 __device__ __forceinline__ void store_slab(float (*Xs)[XLD], float (*Ys)[BN],
-                                           const float4 (&xv)[BM * BK / 4 / NT],
-                                           const float4 (&yv)[BK * BN / 4 / NT],
+                                           const vf4 (&xv)[BM * BK / 4 / NT],
+                                           const vf4 (&yv)[BK * BN / 4 / NT],
                                            int tid) {
   constexpr int XL = BM * BK / 4 / NT, YL = BK * BN / 4 / NT;
 #pragma unroll
@@ -103,7 +108,7 @@ __device__ __forceinline__ void store_slab(float (*Xs)[XLD], float (*Ys)[BN],
     const int f = tid + NT * h;
     const int k = f / (BN / 4);
     const int c = (f % (BN / 4)) * 4;
-    *reinterpret_cast<float4 *>(&Ys[k][c]) = yv[h];
+    *reinterpret_cast<vf4 *>(&Ys[k][c]) = yv[h];
   }
 }
 
@@ -143,7 +148,7 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = Op::identity;
 
-  float4 xv[XL], yv[YL];
+  vf4 xv[XL], yv[YL];
   const int nkt = LIST ? nlist : (K + BK - 1) / BK;
   if (nkt > 0) {
     const int k0 = LIST ? list[0] * BK : 0;

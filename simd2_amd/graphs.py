@@ -69,3 +69,82 @@ def batch_specs(count, seed, nmin=32, nmax=512, pmin=0.01, pmax=0.2, mix=None):
         op, epi, check, _ = mix[kinds[k]]
         out.append((int(ns[k]), op, epi, check, float(ps[k]), int(seed * 100003 + k)))
     return out
+
+
+def _fill_value(op):
+    return {"min_plus": float("inf"), "max_min": float("-inf"),
+            "min_max": float("inf"), "max_plus": float("-inf")}[op]
+
+
+def _finish(torch, op, mask, w, diag=True):
+    """Adjacency of `op` from an edge mask and weights: no edge = the
+    (+)-identity; diagonal 0 for the min-based ops (APSP, MST), left as no
+    edge for max_min (MCP) and max_plus (APLP)."""
+    X = torch.where(mask, w, torch.full_like(w, _fill_value(op)))
+    if diag and op in ("min_plus", "min_max"):
+        X.fill_diagonal_(0.0)
+    return X
+
+
+def er_family(torch, op, n, p, seed, wlo=1, whi=100, device="cuda"):
+    """Directed Erdos-Renyi (symmetric for min_max), integer weights."""
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    mask = torch.rand((n, n), generator=g, device=device) < p
+    w = torch.randint(wlo, whi + 1, (n, n), generator=g, device=device).float()
+    if op == "min_max":
+        mask = torch.triu(mask, 1)
+        mask = mask | mask.T
+        w = torch.triu(w, 1)
+        w = w + w.T
+    mask.fill_diagonal_(False)
+    return _finish(torch, op, mask, w)
+
+
+def grid_family(torch, op, n, seed, diag_p=0.3, wlo=1, whi=1000, device="cuda"):
+    """2D grid of n vertices, floor(sqrt(n)) per row (the last row partial),
+    numbered row by row; each vertex is joined to its right and lower
+    neighbours and, with probability diag_p, to one of its two lower diagonal
+    neighbours; undirected, integer weights in [wlo, whi]."""
+    import math
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    cols = max(1, int(math.isqrt(n)))
+    idx = torch.arange(n, device=device)
+    r, c = idx // cols, idx % cols
+    src, dst = [], []
+    right = (c + 1 < cols) & (idx + 1 < n)
+    src.append(idx[right]); dst.append(idx[right] + 1)
+    down = idx + cols < n
+    src.append(idx[down]); dst.append(idx[down] + cols)
+    pick = torch.rand(n, generator=g, device=device) < diag_p
+    left = torch.rand(n, generator=g, device=device) < 0.5
+    dr = pick & ~left & (c + 1 < cols) & (idx + cols + 1 < n)
+    dl = pick & left & (c > 0) & (idx + cols - 1 < n)
+    src.append(idx[dr]); dst.append(idx[dr] + cols + 1)
+    src.append(idx[dl]); dst.append(idx[dl] + cols - 1)
+    s = torch.cat(src); d = torch.cat(dst)
+    wts = torch.randint(wlo, whi + 1, (s.numel(),), generator=g, device=device).float()
+    mask = torch.zeros((n, n), dtype=torch.bool, device=device)
+    w = torch.zeros((n, n), device=device)
+    mask[s, d] = True; mask[d, s] = True
+    w[s, d] = wts; w[d, s] = wts
+    return _finish(torch, op, mask, w)
+
+
+def rgg_family(torch, op, n, degree, seed, wlo=1, whi=100, device="cuda"):
+    """Random geometric graph: n points uniform in the unit square (vertex ids
+    in the order drawn), joined when closer than the radius that gives the
+    requested mean degree (pi r^2 n = degree); undirected, integer weights."""
+    import math
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    pts = torch.rand((n, 2), generator=g, device=device)
+    rad = math.sqrt(degree / (math.pi * n))
+    d2 = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+    mask = d2 < rad * rad
+    mask.fill_diagonal_(False)
+    w = torch.randint(wlo, whi + 1, (n, n), generator=g, device=device).float()
+    w = torch.triu(w, 1)
+    w = w + w.T
+    return _finish(torch, op, mask, w)

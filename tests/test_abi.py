@@ -12,7 +12,8 @@ from conftest import ROOT
 import simd2_amd
 
 HEADERS = [os.path.join(ROOT, "include", "simd2", h)
-           for h in ("srgemm.h", "converge.h", "tensor_emu.h", "knn.h", "batch.h")]
+           for h in ("srgemm.h", "converge.h", "tensor_emu.h", "knn.h", "batch.h", "schedule.h",
+                     "learned.h")]
 
 
 def declared_functions():
@@ -149,3 +150,41 @@ def test_batch_argument_errors(lib):
     assert L.simd2_batch_upload(None, 0, P1, 8, None) == simd2_amd.ERROR_NULL_POINTER
     assert L.simd2_batch_iterations(None, None, None, None) == simd2_amd.ERROR_NULL_POINTER
     assert L.simd2_batch_destroy(None) == simd2_amd.SUCCESS
+
+
+def test_schedule_and_learned_argument_errors(lib):
+    import simd2_amd.learned as LG
+    import simd2_amd.schedule as S
+    L = LG._bind()
+    itr = ctypes.c_int(-1)
+    sc = S.Schedule(-1, 0)
+    f = L.simd2_converge_scheduled
+    ws = L.simd2_converge_scheduled_workspace(300, -1, 16)
+    assert ws > 2 * 384 * 384 * 4
+    assert L.simd2_converge_scheduled_workspace(0, -1, 16) == 0
+    assert f(1, 300, P1, 300, 1, 1, 16, ctypes.byref(sc), None, P2, ws, None, ctypes.byref(itr),
+             None) == simd2_amd.ERROR_NOT_SUPPORTED            # plus_mul
+    assert f(0, 300, P1, 299, 1, 1, 16, None, None, P2, ws, None, None, None) == \
+        simd2_amd.ERROR_INVALID_PROBLEM
+    assert f(0, 300, P1, 300, 1, 1, 16, None, None, P2, ws - 1, None, None, None) == \
+        simd2_amd.ERROR_WORKSPACE
+    assert f(0, 300, None, 300, 1, 1, 16, None, None, P2, ws, None, None, None) == \
+        simd2_amd.ERROR_NULL_POINTER
+    bad = S.Schedule(7, 0)
+    assert f(0, 300, P1, 300, 1, 1, 16, ctypes.byref(bad), None, P2, ws, None, None, None) == \
+        simd2_amd.ERROR_INVALID_PROBLEM
+    assert L.simd2_graph_features_workspace(40000) == 0
+    fw = L.simd2_graph_features_workspace(100)
+    assert L.simd2_graph_features(0, 100, P1, 100, 1, P2, P3, fw - 1, None) == \
+        simd2_amd.ERROR_WORKSPACE
+    assert L.simd2_graph_features(7, 100, P1, 100, 1, P2, P3, fw, None) == \
+        simd2_amd.ERROR_NOT_SUPPORTED
+    mw = L.simd2_mlp_workspace(64)
+    assert L.simd2_mlp_gradient(P1, P2, P3, P4, 64, None, None, P1, mw - 1, None) == \
+        simd2_amd.ERROR_WORKSPACE
+    assert L.simd2_mlp_train_step(P1, P2, P3, 0, 1e-3, P4, P1, P2, 64, None, P3, mw, None) == \
+        simd2_amd.ERROR_INVALID_PROBLEM                         # steps count from 1
+    lw = L.simd2_converge_learned_workspace(500, 32)
+    assert lw > ws
+    assert L.simd2_converge_learned(0, 500, P1, 500, 1, 1, 32, None, P2, lw, None, None, None,
+                                    None) == simd2_amd.ERROR_NULL_POINTER

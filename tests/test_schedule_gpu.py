@@ -1,0 +1,101 @@
+"""Scheduled converge loop (include/simd2/schedule.h) against the plain loop:
+same matrix bit for bit and the same iteration count, whatever tile shape,
+number of squarings enqueued ahead, or vertex order it is given, while
+tiles and slabs are actually skipped."""
+import pytest
+
+import simd2_amd
+from simd2_amd.graphs import er_family, grid_family, rgg_family
+from simd2_amd.schedule import converge_scheduled
+
+pytestmark = pytest.mark.gpu
+
+OPS = [("min_plus", True, "threshold"), ("max_min", True, "threshold"),
+       ("min_max", False, "threshold"), ("max_plus", True, "aplp")]
+
+
+def family(torch, kind, op, n, seed):
+    if kind == "er_sparse":
+        return er_family(torch, op, n, 3.0 / n, seed)
+    if kind == "er_dense":
+        return er_family(torch, op, n, 0.05, seed)
+    if kind == "grid":
+        return grid_family(torch, op, n, seed)
+    if kind == "rgg":
+        return rgg_family(torch, op, n, 10, seed)
+    raise ValueError(kind)
+
+
+def dag(torch, X):
+    """Longest paths need a DAG: keep the upper triangle only."""
+    return torch.where(torch.triu(torch.ones_like(X, dtype=torch.bool), 1), X,
+                       torch.full_like(X, float("-inf")))
+
+
+def plain(torch, op, X, epi, check, max_itr=64):
+    Y = X.clone()
+    _, itr, st = simd2_amd.converge(op, Y, epilogue=epi, check=check, max_itr=max_itr)
+    return Y, itr, st
+
+
+@pytest.mark.parametrize("kind", ["er_sparse", "er_dense", "grid", "rgg"])
+@pytest.mark.parametrize("n", [100, 257, 640])
+def test_matches_plain_loop(gpu, kind, n):
+    torch = gpu
+    for k, (op, epi, check) in enumerate(OPS):
+        X = family(torch, kind, op, n, 31 * n + k)
+        if op == "max_plus":
+            X = dag(torch, X)
+        Y, itr, st = plain(torch, op, X, epi, check)
+        assert st == simd2_amd.SUCCESS
+        for cfg, pred in ((-1, 0), (0, 3), (1, 64), (2, itr)):
+            print(f"{kind} n={n} {op} cfg={cfg} pred={pred}", flush=True)
+            Z, itr2, st2, rep = converge_scheduled(op, X.clone(), epilogue=epi, check=check,
+                                                   tile_config=cfg, predicted_itr=pred)
+            assert st2 == simd2_amd.SUCCESS and itr2 == itr, (itr2, itr)
+            assert torch.equal(Z.view(torch.int32), Y.view(torch.int32))
+            assert rep["tiles_computed"] <= rep["tiles_total"]
+            if pred >= itr:
+                assert rep["host_syncs"] == 1
+            if pred == 0:
+                assert rep["host_syncs"] == itr
+
+
+def test_skipping_happens_on_banded_graphs(gpu):
+    torch = gpu
+    X = grid_family(torch, "min_plus", 2048, 5)
+    Y, itr, _ = plain(torch, "min_plus", X, True, "threshold")
+    Z, itr2, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=1)
+    assert itr2 == itr and torch.equal(Z, Y)
+    # the early squarings of a banded matrix touch few slabs
+    assert rep["slab_tiles_computed"] < 0.6 * rep["slab_tiles_total"], rep
+
+
+def test_vertex_order_does_not_change_result(gpu):
+    torch = gpu
+    n = 300
+    X = rgg_family(torch, "min_plus", n, 8, 11)
+    Y, itr, _ = plain(torch, "min_plus", X, True, "threshold")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    perm = torch.randperm(n, generator=g, device="cuda").to(torch.int32)
+    Z, itr2, _, _ = converge_scheduled("min_plus", X.clone(), perm=perm)
+    assert itr2 == itr and torch.equal(Z, Y)
+
+
+def test_cap_and_edge_sizes(gpu):
+    torch = gpu
+    n = 200
+    path = torch.full((n, n), float("inf"), device="cuda")
+    i = torch.arange(n - 1, device="cuda")
+    path[i, i + 1] = 1.0
+    path.fill_diagonal_(0.0)
+    Y, itr, st = plain(torch, "min_plus", path, True, "threshold", max_itr=4)
+    Z, itr2, st2, _ = converge_scheduled("min_plus", path.clone(), max_itr=4, predicted_itr=2)
+    assert st == st2 == simd2_amd.ERROR_NOT_CONVERGED and itr == itr2 == 4
+    assert torch.equal(Y, Z)
+    for n in (1, 2, 15, 16, 17, 129):
+        X = er_family(torch, "min_plus", n, 0.3, n)
+        Y, itr, _ = plain(torch, "min_plus", X, True, "threshold")
+        Z, itr2, _, _ = converge_scheduled("min_plus", X.clone(), predicted_itr=5)
+        assert itr2 == itr and torch.equal(Z, Y), n
