@@ -250,6 +250,22 @@ def batched_graphs(torch, simd2_amd, count=4096, seed=0):
             "identical_to_loop": identical, "same_iterations": same_itr}
 
 
+def learned_schedule(torch, count, n=8192):
+    """Learned-schedule APSP / MCP at n = 8192 on the first `count` held-out
+    graphs (ER / grid / geometric in turn): plain converge loop vs the learned
+    loop vs the scheduled loop without the network, every result checked bit
+    for bit against the plain loop."""
+    import os
+    from simd2_amd.eval_schedule import evaluate, summarize
+    from simd2_amd.learned import MODEL_PATH
+    if not os.path.exists(MODEL_PATH):
+        return {"error": f"no trained model at {MODEL_PATH}"}
+    rows = evaluate(torch, count, n)
+    s = summarize(rows)
+    s["n"] = n
+    return s
+
+
 def pld_knn(torch, simd2_amd, Q=16384, R=16384, d=128, k=16, sample=512):
     """PLD/kNN workload: 64 Gaussian clusters (centres ~N(0, 10^2), sigma 1),
     Q queries x R references, d = 128, k = 16.  Times the plain FP32 direct path
@@ -323,6 +339,8 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip 8192^3 and APSP extras")
+    ap.add_argument("--learned-graphs", type=int, default=12,
+                    help="held-out n=8192 graphs for the learned-schedule extra")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the semiring GEMM path runs on a single "
@@ -416,6 +434,9 @@ def main():
         torch.cuda.empty_cache()
         extras["batched_4096"] = batched_graphs(torch, simd2_amd)
         log(f"batched 4096 graphs: {extras['batched_4096']}")
+        torch.cuda.empty_cache()
+        extras["learned_8192"] = learned_schedule(torch, args.learned_graphs)
+        log(f"learned schedule n=8192: {extras['learned_8192']}")
         torch.cuda.empty_cache()
 
     cpu = None

@@ -21,6 +21,7 @@ constexpr int GRAD_CHUNK = 16;  // samples per gradient workgroup
 constexpr int RS = 8;           // row statistics per vertex
 constexpr int BFS_T = 256;      // threads of a breadth-first search
 constexpr int BFS_W = 4;        // adjacency words per thread (n <= 32768)
+constexpr int BFS_U = 16;       // frontier vertices expanded together
 constexpr int NSRC = 8;         // sampled search sources
 
 // An entry is an edge when it is off the diagonal and a finite weight (not
@@ -132,16 +133,17 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
   for (;;) {
     const int cnt = s_cnt[0];
     uint32_t acc[BFS_W] = {0u, 0u, 0u, 0u};
-    for (int base = 0; base < cnt; base += 8) {
-      int u[8];
+    // BFS_U frontier vertices at a time, so that many row loads are in flight
+    for (int base = 0; base < cnt; base += BFS_U) {
+      int u[BFS_U];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) u[k] = base + k < cnt ? cur[base + k] : -1;
+      for (int k = 0; k < BFS_U; ++k) u[k] = base + k < cnt ? cur[base + k] : -1;
 #pragma unroll
       for (int q = 0; q < BFS_W; ++q) {
         const int w = t + BFS_T * q;
         if (w < nw) {
 #pragma unroll
-          for (int k = 0; k < 8; ++k)
+          for (int k = 0; k < BFS_U; ++k)
             if (u[k] >= 0) acc[q] |= adj[(int64_t)u[k] * nw + w];
         }
       }

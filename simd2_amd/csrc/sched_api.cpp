@@ -113,12 +113,10 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   hipError_t e = launch_fill_f32(buf[0], ident, P2, stream);
   if (e == hipSuccess) e = launch_fill_f32(buf[1], ident, P2, stream);
   if (e == hipSuccess) e = launch_permute_copy(buf[0], L.P, X, ldx, n, perm, 0, stream);
-  const int64_t flag_ints = (int64_t)(max_itr + 1) * (L.tiles_r + L.tiles_c + 1);
   if (e == hipSuccess) e = launch_fill_i32(rowchg, 0, (int64_t)(max_itr + 1) * L.tiles_r, stream);
   if (e == hipSuccess) e = launch_fill_i32(colchg, 0, (int64_t)(max_itr + 1) * L.tiles_c, stream);
   if (e == hipSuccess) e = launch_fill_i32(conv, 0, max_itr + 1, stream);
   if (e == hipSuccess) e = launch_fill_i32(reinterpret_cast<int *>(stats), 0, 4, stream);
-  (void)flag_ints;
   if (e == hipSuccess)
     e = launch_sched_occupancy(op, cfg, buf[0], L.P, n, L.tiles_r, L.tiles_c, L.slabs, xocc[0],
                                yocc[0], stream);

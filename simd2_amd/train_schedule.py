@@ -50,8 +50,9 @@ def make_graph(torch, fam, op, n, param, seed):
     return rgg_family(torch, op, n, param, seed)
 
 
-def label_runs(torch, specs, log_every=500, max_itr=64):
-    """Features, squarings and per-tile-shape times of every graph."""
+def label_runs(torch, specs, log_every=500, max_itr=64, checkpoint=None):
+    """Features, squarings and per-tile-shape times of every graph
+    (checkpoint(feats, itrs, times) is called at every progress line)."""
     from .learned import CLASS_OF_TILE, graph_features
     from .schedule import converge_scheduled
     feats, itrs, times = [], [], []
@@ -79,6 +80,9 @@ def label_runs(torch, specs, log_every=500, max_itr=64):
         if (k + 1) % log_every == 0:
             print(f"  labelled {k + 1}/{len(specs)} graphs ({time.time() - t0:.0f} s)",
                   flush=True)
+            if checkpoint:
+                checkpoint(np.array(feats, np.float32), np.array(itrs, np.int32),
+                           np.array(times))
     return np.array(feats, np.float32), np.array(itrs, np.int32), np.array(times)
 
 
@@ -171,10 +175,13 @@ def main(argv=None):
         specs = [graph_spec(rng, args.nmin, args.nmax) for _ in range(args.graphs)]
         print(f"labelling {len(specs)} method-generated runs", flush=True)
         t0 = time.time()
-        feats, itrs, times = label_runs(torch, specs)
+
+        def save(f, i, t):
+            if args.data_out:
+                np.savez_compressed(args.data_out, features=f, squarings=i, tile_times=t)
+        feats, itrs, times = label_runs(torch, specs, checkpoint=save)
         print(f"labelled in {time.time() - t0:.0f} s", flush=True)
-        if args.data_out:
-            np.savez_compressed(args.data_out, features=feats, squarings=itrs, tile_times=times)
+        save(feats, itrs, times)
     tile_cls = times.argmin(1).astype(np.int32)
     t0 = time.time()
     P, norm, history, val = train(torch, feats, itrs, tile_cls, epochs=args.epochs,
