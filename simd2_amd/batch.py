@@ -114,3 +114,45 @@ class Batch:
 
 
 __all__ = ["Batch", "Simd2Error", "SUCCESS"]
+
+
+def _main(argv=None):
+    """python -m simd2_amd.batch [--graphs 4096] [--solves 3] [--tune]: the
+    batched workload (n in [32, 512], p in [0.01, 0.2], 40/20/20/20 mix of
+    min_plus / max_min / min_max / packed or_and), solved --solves times."""
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--graphs", type=int, default=4096)
+    ap.add_argument("--solves", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tune", action="store_true")
+    args = ap.parse_args(argv)
+    torch = _torch()
+    from .graphs import app_graph, batch_specs
+    specs = batch_specs(args.graphs, args.seed)
+    mats = [app_graph(torch, op, n, p, s) for n, op, epi, check, p, s in specs]
+    b = Batch([s[:4] for s in specs])
+    if args.tune:
+        for g, X in enumerate(mats):
+            b.upload(g, X)
+        b.autotune()
+    times = []
+    for _ in range(args.solves):
+        for g, X in enumerate(mats):
+            b.upload(g, X)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.solve()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    itr, bad = b.iterations()
+    cfg, launches = b.schedule()
+    print(json.dumps({"graphs": args.graphs, "solve_ms": [round(t, 3) for t in times],
+                      "squarings": int(itr.sum()), "max_squarings": int(itr.max()),
+                      "not_converged": bad, "launches_per_round": launches}), flush=True)
+
+
+if __name__ == "__main__":
+    _main()
