@@ -32,6 +32,11 @@ typedef struct simd2_schedule {
                         -1: the automatic choice of simd2_pick_tile_config */
   int predicted_itr; /* squarings enqueued before the first flag read;
                         0: read the flag after every squaring */
+  int row_sparse;    /* a squaring whose input holds at most row_sparse * n
+                        entries other than the (+)-identity runs row by row
+                        over those entries (one pass over row k per entry
+                        (i, k)) instead of tile by tile; 0: never; -1: the
+                        default, 128 */
 } simd2_schedule;
 
 typedef struct simd2_sched_report {
@@ -41,6 +46,8 @@ typedef struct simd2_sched_report {
   unsigned long long tiles_computed;       /* tiles this run computed */
   unsigned long long slab_tiles_total;     /* (tile, 16-deep K slab) pairs, plain loop */
   unsigned long long slab_tiles_computed;  /* pairs this run computed */
+  unsigned long long rows_sparse;          /* rows squared row by row */
+  unsigned long long row_entries;          /* row entries those passes visited */
 } simd2_sched_report;
 
 /* Bytes of workspace simd2_converge_scheduled needs (two padded copies of
@@ -54,7 +61,7 @@ size_t simd2_converge_scheduled_workspace(int n, int tile_config, int max_itr);
  * max_plus).  perm: optional device array of n vertex indices; the loop runs
  * on X[perm][perm] and the result is mapped back (a vertex order with
  * locality lets more tiles and slabs be skipped; the result does not depend
- * on it).  schedule: NULL for { -1, 0 }.  Returns like simd2_converge_f32;
+ * on it).  schedule: NULL for { -1, 0, -1 }.  Returns like simd2_converge_f32;
  * *num_itr and *report (both optional) describe the run.  Synchronises
  * `stream`. */
 int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,

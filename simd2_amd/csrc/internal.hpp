@@ -141,11 +141,22 @@ struct SchedArgs {
   const int *conv_prev;         // change-test flag of the previous squaring (null: first)
   int *conv;                    // change-test flag (check mode) of this squaring
   unsigned long long *stats;    // optional: [0] tiles computed, [1] slab-tiles computed
+  // Row-sparse squarings: an input with at most nnz_limit entries other than
+  // the (+)-identity is squared row by row over its entries (one workgroup
+  // per row) instead of tile by tile.  nnz_in: that count for this squaring's
+  // input; nnz_out: for its output (INT_MAX after a tiled squaring, whose
+  // output is never sparser).
+  const int *nnz_in;
+  int *nnz_out;
+  int nnz_limit;
 };
 hipError_t launch_sched_square(int op, int cfg, const SchedArgs &a, hipStream_t s);
+// After a row-sparse squaring (nothing otherwise): occupancy maps and entry
+// count of its output.  With a == null: of `D` for the first squaring.
 hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, int n,
                                   int tiles_r, int tiles_c, int slabs, uint8_t *xocc,
-                                  uint8_t *yocc, hipStream_t s);
+                                  uint8_t *yocc, int *nnz_out, const SchedArgs *a,
+                                  hipStream_t s);
 hipError_t launch_fill_f32(float *p, float v, int64_t n, hipStream_t s);
 hipError_t launch_permute_copy(float *dst, int64_t ldd, const float *src, int64_t lds, int n,
                                const int *perm, int inverse, hipStream_t s);
@@ -154,8 +165,10 @@ hipError_t launch_permute_copy(float *dst, int64_t ldd, const float *src, int64_
 size_t graph_features_workspace(int n);
 hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int epi, float *f,
                                  void *ws, hipStream_t s);
+// after_features: launch_graph_features already ran on this workspace (its
+// packed adjacency and search from vertex 0 are reused).
 hipError_t launch_graph_levels(int n, const float *X, int64_t ld, int *level, void *ws,
-                               hipStream_t s);
+                               bool after_features, hipStream_t s);
 int mlp_param_count();
 size_t mlp_workspace(int B);
 hipError_t launch_mlp_forward(const float *P, const float *x, int B, float *y, const float *norm,

@@ -21,7 +21,7 @@ constexpr int GRAD_CHUNK = 16;  // samples per gradient workgroup
 constexpr int RS = 8;           // row statistics per vertex
 constexpr int BFS_T = 256;      // threads of a breadth-first search
 constexpr int BFS_W = 4;        // adjacency words per thread (n <= 32768)
-constexpr int BFS_U = 16;       // frontier vertices expanded together
+constexpr int BFS_U = 32;       // frontier vertices expanded together
 constexpr int NSRC = 8;         // sampled search sources
 
 // An entry is an edge when it is off the diagonal and a finite weight (not
@@ -112,9 +112,10 @@ __global__ __launch_bounds__(256) void pack_adj_kernel(const float *X, int64_t l
 // Breadth-first search from src over the packed adjacency, by one
 // workgroup.  The frontier is a vertex list in global memory (cur / nxt, n
 // ints each); thread t owns adjacency words t + 256 q.  Writes depth (the
-// last level reached) and reached (vertex count) to res[0..1] and, when
-// level is set, each vertex's level (-1: unreached).  Every level adds at
-// least one vertex or ends the search, so it stops within n levels.
+// last level reached), reached (vertex count) and the lowest-numbered vertex
+// of the last level to res[0..2] and, when level is set, each vertex's level
+// (-1: unreached).  Every level adds at least one vertex or ends the search,
+// so it stops within n levels.  s_cnt: 3 ints of LDS.
 __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *nxt, int *level,
                     int *res, uint32_t *visited, int *s_cnt) {
   const int t = tid_x();
@@ -125,11 +126,12 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
   if (t == 0) {
     visited[src / 32] |= 1u << (src % 32);
     cur[0] = src;
-    s_cnt[0] = 1;  // frontier size
-    s_cnt[1] = 0;  // next frontier size
+    s_cnt[0] = 1;      // frontier size
+    s_cnt[1] = 0;      // next frontier size
+    s_cnt[2] = 0x7fffffff;  // lowest vertex of the next frontier
   }
   __syncthreads();
-  int depth = 0, reached = 1;
+  int depth = 0, reached = 1, far = src;
   for (;;) {
     const int cnt = s_cnt[0];
     uint32_t acc[BFS_W] = {0u, 0u, 0u, 0u};
@@ -155,6 +157,7 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
         uint32_t fresh = acc[q] & ~visited[w];
         if (fresh) {
           visited[w] |= fresh;
+          atomicMin(&s_cnt[2], w * 32 + __ffs(fresh) - 1);
           int at = atomicAdd(&s_cnt[1], __popc(fresh));
           while (fresh) {
             const int b = __ffs(fresh) - 1;
@@ -168,22 +171,26 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
     }
     __syncthreads();
     const int ncnt = s_cnt[1];
+    const int nfar = s_cnt[2];
     __syncthreads();
     if (ncnt == 0) break;
     ++depth;
     reached += ncnt;
+    far = nfar;
     int *tmp = cur;
     cur = nxt;
     nxt = tmp;
     if (t == 0) {
       s_cnt[0] = ncnt;
       s_cnt[1] = 0;
+      s_cnt[2] = 0x7fffffff;
     }
     __syncthreads();
   }
   if (t == 0) {
     res[0] = depth;
     res[1] = reached;
+    res[2] = far;
   }
   __syncthreads();
 }
@@ -192,31 +199,30 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
 __global__ __launch_bounds__(BFS_T) void bfs_sources_kernel(const uint32_t *adj, int n, int nw,
                                                             int *lists, int *res) {
   __shared__ uint32_t visited[BFS_T * BFS_W];
-  __shared__ int s_cnt[2];
+  __shared__ int s_cnt[3];
   const int k = bid_x();
   const int src = (int)((int64_t)k * n / NSRC);
   int *cur = lists + (int64_t)k * 2 * n;
-  bfs(adj, n, nw, src, cur, cur + n, nullptr, res + 2 * k, visited, s_cnt);
+  bfs(adj, n, nw, src, cur, cur + n, nullptr, res + 3 * k, visited, s_cnt);
 }
 
-// Vertex levels for the order: search from vertex 0, restart from the
-// lowest-numbered vertex of its last level (a pseudo-peripheral vertex) and
-// keep the levels of the second search.
+// Vertex levels for the order: the search from the lowest-numbered vertex of
+// the last level of a search from vertex 0 (a pseudo-peripheral vertex).
+// far: that vertex when already known (the feature searches start one at
+// vertex 0), else null and the first search runs here.
 __global__ __launch_bounds__(BFS_T) void bfs_levels_kernel(const uint32_t *adj, int n, int nw,
-                                                           int *lists, int *level, int *res) {
+                                                           int *lists, int *level, int *res,
+                                                           const int *far) {
   __shared__ uint32_t visited[BFS_T * BFS_W];
-  __shared__ int s_cnt[2];
-  __shared__ int s_far;
-  bfs(adj, n, nw, 0, lists, lists + n, level, res, visited, s_cnt);
-  const int depth = res[0];
-  if (tid_x() == 0) s_far = n;
-  __syncthreads();
-  int far = n;
-  for (int v = tid_x(); v < n; v += BFS_T)
-    if (level[v] == depth) far = min(far, v);
-  if (far < n) atomicMin(&s_far, far);
-  __syncthreads();
-  bfs(adj, n, nw, s_far, lists, lists + n, level, res + 2, visited, s_cnt);
+  __shared__ int s_cnt[3];
+  int src;
+  if (far) {
+    src = *far;
+  } else {
+    bfs(adj, n, nw, 0, lists, lists + n, nullptr, res, visited, s_cnt);
+    src = res[2];
+  }
+  bfs(adj, n, nw, src, lists, lists + n, level, res + 3, visited, s_cnt);
 }
 
 // The 24 features (one workgroup; fixed-order reductions in double).
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(256) void features_kernel(const double *rstats, con
   const double wsd = m > 0 ? sqrt(fmax(sh[5][0] / m - wmean * wmean, 0.0)) : 0.0;
   double dmean = 0, dmx = 0, reach = 0, width = 0;
   for (int k = 0; k < NSRC; ++k) {
-    const double d = bfs_res[2 * k], r = bfs_res[2 * k + 1];
+    const double d = bfs_res[3 * k], r = bfs_res[3 * k + 1];
     dmean += d / NSRC;
     dmx = fmax(dmx, d);
     reach += r / N / NSRC;
@@ -439,7 +445,7 @@ FeatLayout feat_layout(int n) {
   L.sym = take(2 * sizeof(int));
   L.adj = take((size_t)n * nw * sizeof(uint32_t));
   L.lists = take((size_t)NSRC * 2 * n * sizeof(int));
-  L.res = take(2 * NSRC * sizeof(int));
+  L.res = take((3 * NSRC + 6) * sizeof(int));
   L.level = take((size_t)n * sizeof(int));
   L.total = off;
   return L;
@@ -485,15 +491,21 @@ hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int 
 }
 
 hipError_t launch_graph_levels(int n, const float *X, int64_t ld, int *level, void *ws,
-                               hipStream_t s) {
+                               bool after_features, hipStream_t s) {
   const FeatLayout L = feat_layout(n);
   char *w = static_cast<char *>(ws);
   uint32_t *adj = reinterpret_cast<uint32_t *>(w + L.adj);
   int *lists = reinterpret_cast<int *>(w + L.lists);
   int *res = reinterpret_cast<int *>(w + L.res);
-  hipError_t e = pack_adjacency(X, ld, n, adj, s);
-  if (e != hipSuccess) return e;
-  bfs_levels_kernel<<<dim3(1), dim3(BFS_T), 0, s>>>(adj, n, (n + 31) / 32, lists, level, res);
+  // After launch_graph_features on the same workspace the adjacency is packed
+  // and the search from vertex 0 (source 0) has found the far vertex.
+  if (!after_features) {
+    hipError_t e = pack_adjacency(X, ld, n, adj, s);
+    if (e != hipSuccess) return e;
+  }
+  bfs_levels_kernel<<<dim3(1), dim3(BFS_T), 0, s>>>(adj, n, (n + 31) / 32, lists, level,
+                                                    res + 3 * NSRC,
+                                                    after_features ? res + 2 : nullptr);
   return hipGetLastError();
 }
 

@@ -38,11 +38,11 @@ size_t features_bytes(int n) { return align256(graph_features_workspace(n)); }
 
 // Levels of the pseudo-peripheral breadth-first search -> vertex order
 // (unreached vertices last), computed with the features workspace.
-int order_into(int n, const float *X, int ldx, int *perm_dev, void *ws, hipStream_t s,
-               std::vector<int> *perm_host) {
+int order_into(int n, const float *X, int ldx, int *perm_dev, void *ws, bool after_features,
+               hipStream_t s, std::vector<int> *perm_host) {
   char *w = static_cast<char *>(ws);
   int *level = reinterpret_cast<int *>(w + features_bytes(n));
-  hipError_t e = launch_graph_levels(n, X, ldx, level, ws, s);
+  hipError_t e = launch_graph_levels(n, X, ldx, level, ws, after_features, s);
   std::vector<int> lv(n);
   if (e == hipSuccess)
     e = hipMemcpyAsync(lv.data(), level, n * sizeof(int), hipMemcpyDeviceToHost, s);
@@ -83,7 +83,7 @@ int simd2_graph_order(int op, int n, const float *X, int ldx, int *perm, void *w
   if (n <= 0 || n > MAX_N || ldx < n) return SIMD2_ERROR_INVALID_PROBLEM;
   if (!X || !perm || !workspace) return SIMD2_ERROR_NULL_POINTER;
   if (workspace_bytes < simd2_graph_features_workspace(n)) return SIMD2_ERROR_WORKSPACE;
-  return order_into(n, X, ldx, perm, workspace, stream, nullptr);
+  return order_into(n, X, ldx, perm, workspace, false, stream, nullptr);
 }
 
 int simd2_mlp_forward(const float *params, const float *x, int B, float *y, hipStream_t stream) {
@@ -168,18 +168,21 @@ int simd2_converge_learned(int op, int n, float *X, int ldx, int do_epilogue, in
   // Schedule from the prediction.  Enqueueing a squaring too many costs a
   // launch (it returns at once); one too few costs a flag read per extra
   // squaring, hence the rounding up.
-  simd2_schedule sc;
+  simd2_schedule sc{-1, 0, -1};
   const float itr_pred = std::isfinite(host[0]) ? host[0] : 1.0f;
   sc.predicted_itr = std::max(1, std::min(max_itr, (int)std::ceil(itr_pred + 0.25f)));
   const float cls = std::isfinite(host[1]) ? host[1] : 2.0f;
   sc.tile_config = CFG_OF_CLASS[std::max(0, std::min(2, (int)std::lround(cls)))];
   // Vertex order: breadth-first levels when the given numbering has no
-  // locality (mean |i - j| over edges above n / 20) and one search reaches
-  // most of the graph; a graph already numbered with locality keeps it.
+  // locality (mean |i - j| over edges above n / 20), one search reaches most
+  // of the graph, and the graph is "long" (mean search depth above log2 n): a
+  // small-world graph (Erdos-Renyi) has no order with locality, and a graph
+  // already numbered with locality keeps it.
   const float *f = host + SIMD2_MLP_OUT;
-  const bool reorder = f[14] > 0.05f && f[19] > 0.5f;
+  const double depth = std::exp2(16.0 * f[17]) - 1.0;
+  const bool reorder = f[14] > 0.05f && f[19] > 0.5f && depth > std::log2((double)n);
   int st = SIMD2_SUCCESS;
-  if (reorder) st = order_into(n, X, ldx, perm, ws, stream, nullptr);
+  if (reorder) st = order_into(n, X, ldx, perm, ws, true, stream, nullptr);
   if (st != SIMD2_SUCCESS) return st;
   if (used) *used = sc;
   st = simd2_converge_scheduled(op, n, X, ldx, do_epilogue, check_mode, max_itr, &sc,

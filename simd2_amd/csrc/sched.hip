@@ -46,7 +46,9 @@ __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) 
   // A squaring after one that changed nothing computes nothing: the result
   // stays where that squaring left it.
   if (a.conv_prev && *a.conv_prev == 0) return;
+  if (a.nnz_in && *a.nnz_in <= a.nnz_limit) return;  // squared row by row instead
   const int tid = tid_x();
+  if (a.nnz_out && bid_x() == 0 && tid == 0) *a.nnz_out = 0x7fffffff;  // not sparse any more
   int tr, tc;
   map_tile(bid_x(), a.tiles_r, a.tiles_c, tr, tc);
   if (a.rowchg_prev && !a.rowchg_prev[tr] && !a.colchg_prev[tc]) return;  // clean tile
@@ -152,33 +154,135 @@ __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) 
   }
 }
 
-// Occupancy of a matrix (used for the first squaring): one workgroup per
-// bm x bn tile, the same maps the squaring's epilogue writes.
+// Row-sparse squaring, one workgroup per row i:
+//   D'[i][c] = [D[i][c] (+)] (+)_{k : D[i][k] != identity} D[i][k] (x) D[k][c].
+// The terms left out are identity (x) y = identity, so the result is the tiled
+// kernel's bit for bit; it costs one pass over row k per entry of row i, which
+// beats the tiled kernel while rows hold few entries (early squarings of
+// sparse graphs).  Row i's entries are compacted into LDS (wave ballots) in
+// passes of RL_CAP columns; each thread keeps RL_Q output columns in
+// registers, 8192 columns per chunk.
+constexpr int RL_Q = 32;
+constexpr int RL_CHUNK = EW * RL_Q;
+constexpr int RL_CAP = 2048;
+
+template <class Op>
+__global__ __launch_bounds__(EW) void sched_rows_kernel(SchedArgs a, int bm, int bn) {
+  __shared__ uint16_t lk[RL_CAP];
+  __shared__ float lx[RL_CAP];
+  __shared__ int s_cnt;
+  __shared__ int s_wave[EW / 64];
+  if (a.conv_prev && *a.conv_prev == 0) return;
+  if (!a.nnz_in || *a.nnz_in > a.nnz_limit) return;  // squared tile by tile instead
+  const int i = bid_x();
+  const int tid = tid_x(), lane = tid % 64, wave = tid / 64;
+  const uint32_t ident = __float_as_uint(Op::identity);
+  const float *rowi = a.cur + (int64_t)i * a.ld;
+  float *outi = a.nxt + (int64_t)i * a.ld;
+  bool chg_bits = false, chg_test = false;
+  long long entries = 0;
+  for (int c0 = 0; c0 < a.n; c0 += RL_CHUNK) {
+    float acc[RL_Q];
+#pragma unroll
+    for (int q = 0; q < RL_Q; ++q) acc[q] = Op::identity;
+    for (int k0 = 0; k0 < a.n; k0 += RL_CAP) {
+      if (tid == 0) s_cnt = 0;
+      __syncthreads();
+      const int k1 = min(a.n, k0 + RL_CAP);
+      for (int kb = k0; kb < k1; kb += EW) {
+        const int k = kb + tid;
+        const float x = k < k1 ? rowi[k] : Op::identity;
+        const bool p = k < k1 && __float_as_uint(x) != ident;
+        const uint64_t m = __ballot(p);
+        if (lane == 0) s_wave[wave] = __popcll(m);
+        __syncthreads();
+        int off = s_cnt;
+        for (int w = 0; w < wave; ++w) off += s_wave[w];
+        if (p) {
+          const int at = off + __popcll(m & ((1ull << lane) - 1ull));
+          lk[at] = (uint16_t)k;
+          lx[at] = x;
+        }
+        __syncthreads();
+        if (tid == 0) s_cnt += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        __syncthreads();
+      }
+      const int cnt = s_cnt;
+      entries += cnt;
+      for (int e = 0; e < cnt; ++e) {
+        const float x = lx[e];
+        const float *rowk = a.cur + (int64_t)lk[e] * a.ld + c0 + tid;
+#pragma unroll
+        for (int q = 0; q < RL_Q; ++q)
+          if (c0 + tid + EW * q < a.ld) acc[q] = Op::add(acc[q], Op::mul(x, rowk[EW * q]));
+      }
+      __syncthreads();  // the list is rewritten by the next pass
+    }
+#pragma unroll
+    for (int q = 0; q < RL_Q; ++q) {
+      const int c = c0 + tid + EW * q;
+      bool col_chg = false;
+      if (c < a.n) {
+        const float p = rowi[c];
+        float v = acc[q];
+        if (a.epi) v = Op::add(v, p);
+        if (a.mode == CHECK_APLP && v > -1.0f && v < 1.0f) v = -FLT_MAX;
+        col_chg = __float_as_uint(v) != __float_as_uint(p);
+        chg_bits |= col_chg;
+        chg_test |= differs(v, p, a.mode);
+        outi[c] = v;
+      }
+      // a wave covers 64 consecutive columns, inside one column block
+      if (__any(col_chg) && lane == 0) a.colchg[(c0 + EW * q + 64 * wave) / bn] = 1;
+    }
+  }
+  if (__any(chg_bits) && lane == 0) a.rowchg[i / bm] = 1;
+  if (__any(chg_test) && lane == 0) *a.conv = 1;
+  if (tid == 0 && a.stats) {
+    atomicAdd(a.stats + 2, 1ull);                               // rows squared row-sparse
+    atomicAdd(a.stats + 3, (unsigned long long)entries);        // row entries visited
+  }
+}
+
+// Occupancy maps of a matrix, one workgroup per bm x bn tile (the maps the
+// tiled squaring's epilogue writes), and its count of entries other than the
+// identity.  Used for the first squaring's input and after row-sparse
+// squarings; after a tiled squaring (or once converged) it does nothing.
 __global__ __launch_bounds__(EW) void occupancy_kernel(const float *D, int64_t ld, int n, int bm,
                                                        int bn, int tiles_c, int slabs,
                                                        uint32_t ident, uint8_t *xocc,
-                                                       uint8_t *yocc) {
+                                                       uint8_t *yocc, int *nnz_out,
+                                                       const int *conv_prev, const int *nnz_in,
+                                                       int nnz_limit) {
   __shared__ unsigned s_bits[2];
+  __shared__ int s_count;
+  if (conv_prev && *conv_prev == 0) return;
+  if (nnz_in && *nnz_in > nnz_limit) return;
   const int tid = tid_x();
   const int tr = bid_x() / tiles_c, tc = bid_x() % tiles_c;
   const int row0 = tr * bm, col0 = tc * bn;
   if (tid == 0) {
     s_bits[0] = 0;
     s_bits[1] = 0;
+    s_count = 0;
   }
   __syncthreads();
   unsigned xb = 0, yb = 0;
+  int count = 0;
   for (int e = tid; e < bm * bn; e += EW) {
     const int rl = e / bn, cl = e % bn;
     const int r = row0 + rl, c = col0 + cl;
     if (r < n && c < n && __float_as_uint(D[(int64_t)r * ld + c]) != ident) {
       xb |= 1u << (cl / 16);
       yb |= 1u << (rl / 16);
+      ++count;
     }
   }
   if (xb) atomicOr(&s_bits[0], xb);
   if (yb) atomicOr(&s_bits[1], yb);
+  if (count) atomicAdd(&s_count, count);
   __syncthreads();
+  if (tid == 0 && nnz_out && s_count) atomicAdd(nnz_out, s_count);
   const int xs = bn / 16, ys = bm / 16;
   if (tid < xs) {
     const int s = col0 / 16 + tid;
@@ -214,6 +318,10 @@ int grid_for(int64_t n) { return (int)std::min<int64_t>((n + EW - 1) / EW, 4096)
 template <class Op>
 hipError_t sched_op(int cfg, const SchedArgs &a, hipStream_t s) {
   const dim3 grid(a.tiles_r * a.tiles_c), block(256);
+  if (a.nnz_in) {  // the row-sparse form; exactly one of the two kernels works
+    const ValuCfgInfo c = valu_cfg_info(cfg);
+    sched_rows_kernel<Op><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
+  }
   if (cfg == CFG_S64)
     sched_kernel<Op, CfgS><<<grid, block, 0, s>>>(a);
   else if (cfg == CFG_M128x64)
@@ -237,13 +345,15 @@ hipError_t launch_sched_square(int op, int cfg, const SchedArgs &a, hipStream_t 
 
 hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, int n,
                                   int tiles_r, int tiles_c, int slabs, uint8_t *xocc,
-                                  uint8_t *yocc, hipStream_t s) {
+                                  uint8_t *yocc, int *nnz_out, const SchedArgs *a,
+                                  hipStream_t s) {
   const ValuCfgInfo c = valu_cfg_info(cfg);
   const float ident = (op == 0 || op == 6) ? SIMD2_INF : -SIMD2_INF;
   uint32_t bits;
   std::memcpy(&bits, &ident, 4);
-  occupancy_kernel<<<dim3(tiles_r * tiles_c), dim3(EW), 0, s>>>(D, ld, n, c.bm, c.bn, tiles_c,
-                                                                slabs, bits, xocc, yocc);
+  occupancy_kernel<<<dim3(tiles_r * tiles_c), dim3(EW), 0, s>>>(
+      D, ld, n, c.bm, c.bn, tiles_c, slabs, bits, xocc, yocc, nnz_out,
+      a ? a->conv_prev : nullptr, a ? a->nnz_in : nullptr, a ? a->nnz_limit : 0);
   return hipGetLastError();
 }
 

@@ -31,7 +31,7 @@ bool sched_op_supported(int op) {
 // Workspace carve-up (every piece 256-byte aligned).
 struct Layout {
   int P, kpad, tiles_r, tiles_c, slabs;
-  size_t buf[2], xocc[2], yocc[2], rowchg, colchg, conv, stats, total;
+  size_t buf[2], xocc[2], yocc[2], rowchg, colchg, conv, nnz, stats, total;
 };
 
 bool layout(int n, int cfg, int max_itr, Layout *L) {
@@ -60,7 +60,8 @@ bool layout(int n, int cfg, int max_itr, Layout *L) {
   L->rowchg = take(rounds * L->tiles_r * sizeof(int));
   L->colchg = take(rounds * L->tiles_c * sizeof(int));
   L->conv = take(rounds * sizeof(int));
-  L->stats = take(2 * sizeof(unsigned long long));
+  L->nnz = take((rounds + 1) * sizeof(int));
+  L->stats = take(4 * sizeof(unsigned long long));
   L->total = off;
   return true;
 }
@@ -87,9 +88,12 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   if (n == 0) return SIMD2_SUCCESS;
   if (ldx < n || n > 16 * SCHED_MAX_SLABS) return SIMD2_ERROR_INVALID_PROBLEM;
   if (!X || !workspace) return SIMD2_ERROR_NULL_POINTER;
-  simd2_schedule sc = schedule ? *schedule : simd2_schedule{-1, 0};
+  simd2_schedule sc = schedule ? *schedule : simd2_schedule{-1, 0, -1};
   if (sc.tile_config >= CFG_COUNT || sc.predicted_itr < 0) return SIMD2_ERROR_INVALID_PROBLEM;
   const int cfg = sc.tile_config >= 0 ? sc.tile_config : valu_pick_cfg(op, n, n, n);
+  // row-sparse squarings up to `row_sparse` entries per row on average
+  const int per_row = sc.row_sparse < 0 ? 128 : std::min(sc.row_sparse, n);
+  const bool rows_on = per_row > 0;
   Layout L;
   if (!layout(n, cfg, max_itr, &L)) return SIMD2_ERROR_INVALID_PROBLEM;
   if (workspace_bytes < L.total) return SIMD2_ERROR_WORKSPACE;
@@ -104,6 +108,7 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   int *rowchg = reinterpret_cast<int *>(ws + L.rowchg);
   int *colchg = reinterpret_cast<int *>(ws + L.colchg);
   int *conv = reinterpret_cast<int *>(ws + L.conv);
+  int *nnz = reinterpret_cast<int *>(ws + L.nnz);
   unsigned long long *stats = reinterpret_cast<unsigned long long *>(ws + L.stats);
   const float inf = __builtin_inff();
   const float ident = (op == SIMD2_MIN_PLUS || op == SIMD2_MIN_MAX) ? inf : -inf;
@@ -116,10 +121,11 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   if (e == hipSuccess) e = launch_fill_i32(rowchg, 0, (int64_t)(max_itr + 1) * L.tiles_r, stream);
   if (e == hipSuccess) e = launch_fill_i32(colchg, 0, (int64_t)(max_itr + 1) * L.tiles_c, stream);
   if (e == hipSuccess) e = launch_fill_i32(conv, 0, max_itr + 1, stream);
-  if (e == hipSuccess) e = launch_fill_i32(reinterpret_cast<int *>(stats), 0, 4, stream);
+  if (e == hipSuccess) e = launch_fill_i32(nnz, 0, max_itr + 2, stream);
+  if (e == hipSuccess) e = launch_fill_i32(reinterpret_cast<int *>(stats), 0, 8, stream);
   if (e == hipSuccess)
     e = launch_sched_occupancy(op, cfg, buf[0], L.P, n, L.tiles_r, L.tiles_c, L.slabs, xocc[0],
-                               yocc[0], stream);
+                               yocc[0], nnz, nullptr, stream);
   if (e != hipSuccess) return hip_status(e);
 
   int t = 0, T = -1, syncs = 0;
@@ -151,7 +157,13 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
       a.conv_prev = s > 0 ? conv + s - 1 : nullptr;
       a.conv = conv + s;
       a.stats = stats;
+      a.nnz_in = rows_on ? nnz + s : nullptr;
+      a.nnz_out = rows_on ? nnz + s + 1 : nullptr;
+      a.nnz_limit = per_row * n;
       e = launch_sched_square(op, cfg, a, stream);
+      if (e == hipSuccess && rows_on)  // maps and count after a row-sparse squaring
+        e = launch_sched_occupancy(op, cfg, a.nxt, L.P, n, L.tiles_r, L.tiles_c, L.slabs,
+                                   a.xocc_out, a.yocc_out, a.nnz_out, &a, stream);
     }
     if (e != hipSuccess) return hip_status(e);
     flags.resize(m);
@@ -171,7 +183,7 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   const int done = T >= 0 ? T + 1 : t;  // squarings whose result counts
   float *result = buf[done & 1];
   e = launch_permute_copy(X, ldx, result, L.P, n, perm, 1, stream);
-  unsigned long long st[2] = {0, 0};
+  unsigned long long st[4] = {0, 0, 0, 0};
   if (e == hipSuccess && report)
     e = hipMemcpyAsync(st, stats, sizeof(st), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -184,6 +196,8 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
     report->tiles_computed = st[0];
     report->slab_tiles_total = tiles * L.slabs * done;
     report->slab_tiles_computed = st[1];
+    report->rows_sparse = st[2];
+    report->row_entries = st[3];
   }
   return T >= 0 ? SIMD2_SUCCESS : SIMD2_ERROR_NOT_CONVERGED;
 }

@@ -49,6 +49,14 @@ def held_out_specs(count, seed=HELD_OUT_SEED):
     return out
 
 
+def reorder_rule(f, n):
+    """The vertex-order rule of simd2_converge_learned: relabel by search
+    levels when the numbering has no locality, a search reaches most of the
+    graph and the mean search depth exceeds log2 n."""
+    depth = 2.0 ** (16.0 * float(f[17])) - 1.0
+    return bool(f[14] > 0.05 and f[19] > 0.5 and depth > math.log2(n))
+
+
 def _timed(torch, fn):
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -86,7 +94,7 @@ def evaluate(torch, count=200, n=8192, model=None, log=None, max_itr=64):
 
         def method():
             f = graph_features(op, W, epi).cpu().numpy()
-            perm = graph_order(op, W) if (f[14] > 0.05 and f[19] > 0.5) else None
+            perm = graph_order(op, W) if reorder_rule(f, n) else None
             return converge_scheduled(op, W, epilogue=epi, check=check, max_itr=max_itr,
                                       perm=perm, workspace=ws_s)
         t_meth, (_, itr_m, st_m, rep_m) = _timed(torch, method)

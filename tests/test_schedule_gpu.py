@@ -48,13 +48,20 @@ def test_matches_plain_loop(gpu, kind, n):
             X = dag(torch, X)
         Y, itr, st = plain(torch, op, X, epi, check)
         assert st == simd2_amd.SUCCESS
-        for cfg, pred in ((-1, 0), (0, 3), (1, 64), (2, itr)):
-            print(f"{kind} n={n} {op} cfg={cfg} pred={pred}", flush=True)
+        # (tile shape, squarings enqueued ahead, row-sparse limit): the default
+        # limit, tiles only, and every squaring row by row
+        for cfg, pred, rs in ((-1, 0, -1), (0, 3, 0), (1, 64, n), (2, itr, -1)):
+            print(f"{kind} n={n} {op} cfg={cfg} pred={pred} row_sparse={rs}", flush=True)
             Z, itr2, st2, rep = converge_scheduled(op, X.clone(), epilogue=epi, check=check,
-                                                   tile_config=cfg, predicted_itr=pred)
+                                                   tile_config=cfg, predicted_itr=pred,
+                                                   row_sparse=rs)
             assert st2 == simd2_amd.SUCCESS and itr2 == itr, (itr2, itr)
             assert torch.equal(Z.view(torch.int32), Y.view(torch.int32))
             assert rep["tiles_computed"] <= rep["tiles_total"]
+            if rs == n:
+                assert rep["rows_sparse"] == n * itr and rep["tiles_computed"] == 0
+            if rs == 0:
+                assert rep["rows_sparse"] == 0
             if pred >= itr:
                 assert rep["host_syncs"] == 1
             if pred == 0:
@@ -65,10 +72,13 @@ def test_skipping_happens_on_banded_graphs(gpu):
     torch = gpu
     X = grid_family(torch, "min_plus", 2048, 5)
     Y, itr, _ = plain(torch, "min_plus", X, True, "threshold")
-    Z, itr2, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=1)
+    Z, itr2, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=1, row_sparse=0)
     assert itr2 == itr and torch.equal(Z, Y)
     # the early squarings of a banded matrix touch few slabs
     assert rep["slab_tiles_computed"] < 0.6 * rep["slab_tiles_total"], rep
+    # with row-sparse squarings the first ones go row by row
+    Z, itr3, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=1)
+    assert itr3 == itr and torch.equal(Z, Y) and rep["rows_sparse"] >= 2048, rep
 
 
 def test_vertex_order_does_not_change_result(gpu):
