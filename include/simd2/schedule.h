@@ -2,12 +2,18 @@
  * simd2 -- scheduled repeated squaring (C ABI of libsimd2.so).
  *
  * The apps' converge loop (apps/apsp/cuASR/apsp_cuASR.cu:37-54, the same
- * shape in aplp / mcp / mst: square, compare, read the flag, repeat) with two
- * kinds of work left out, both provably without effect on the result:
+ * shape in aplp / mcp / mst: square, compare, read the flag, repeat) with
+ * work left out that provably has no effect on the result:
  *   - tiles whose row panel and column panel did not change in the previous
  *     squaring (their result is the value they already hold);
  *   - 16-deep K slabs that hold only the (+)-identity in a tile's rows or
  *     columns (their terms are identity (x) y = identity);
+ *   - while the matrix is sparse, every term with an identity operand: the
+ *     squaring runs row by row over the non-identity entries;
+ *   - with the epilogue on, every term whose two operands kept their bits in
+ *     the previous squaring (it is already inside the entry it would update:
+ *     D = D_prev (+) D_prev (x) D_prev and (+) is min or max), when few
+ *     entries changed; that squaring runs row by row over the changed ones;
  * and with the flag read deferred: `predicted_itr` squarings are enqueued
  * before the host first reads the change flags.  Squarings enqueued after
  * the one that changed nothing return at once on the device, so the matrix
@@ -36,7 +42,11 @@ typedef struct simd2_schedule {
                         entries other than the (+)-identity runs row by row
                         over those entries (one pass over row k per entry
                         (i, k)) instead of tile by tile; 0: never; -1: the
-                        default, 128 */
+                        default, 128; at most 1024 */
+  int changed_sparse; /* a squaring after one that changed at most
+                        changed_sparse * n entries runs over those entries
+                        only (needs do_epilogue and check_mode != APLP);
+                        0: never; -1: the default, 64; at most 1024 */
 } simd2_schedule;
 
 typedef struct simd2_sched_report {
@@ -48,6 +58,8 @@ typedef struct simd2_sched_report {
   unsigned long long slab_tiles_computed;  /* pairs this run computed */
   unsigned long long rows_sparse;          /* rows squared row by row */
   unsigned long long row_entries;          /* row entries those passes visited */
+  unsigned long long changed_rows;         /* rows squared over changed entries */
+  unsigned long long changed_entries;      /* changed entries of their own rows */
 } simd2_sched_report;
 
 /* Bytes of workspace simd2_converge_scheduled needs (two padded copies of
@@ -61,7 +73,7 @@ size_t simd2_converge_scheduled_workspace(int n, int tile_config, int max_itr);
  * max_plus).  perm: optional device array of n vertex indices; the loop runs
  * on X[perm][perm] and the result is mapped back (a vertex order with
  * locality lets more tiles and slabs be skipped; the result does not depend
- * on it).  schedule: NULL for { -1, 0, -1 }.  Returns like simd2_converge_f32;
+ * on it).  schedule: NULL for { -1, 0, -1, -1 }.  Returns like simd2_converge_f32;
  * *num_itr and *report (both optional) describe the run.  Synchronises
  * `stream`. */
 int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,

@@ -140,23 +140,39 @@ struct SchedArgs {
   int *colchg;
   const int *conv_prev;         // change-test flag of the previous squaring (null: first)
   int *conv;                    // change-test flag (check mode) of this squaring
-  unsigned long long *stats;    // optional: [0] tiles computed, [1] slab-tiles computed
-  // Row-sparse squarings: an input with at most nnz_limit entries other than
-  // the (+)-identity is squared row by row over its entries (one workgroup
-  // per row) instead of tile by tile.  nnz_in: that count for this squaring's
-  // input; nnz_out: for its output (INT_MAX after a tiled squaring, whose
-  // output is never sparser).
-  const int *nnz_in;
-  int *nnz_out;
-  int nnz_limit;
+  unsigned long long *stats;    // optional: [0] tiles computed, [1] slab-tiles computed,
+                                //   [2] / [3] rows / entries of row-sparse squarings,
+                                //   [4] / [5] rows / entries of changed-entry squarings
+  // Sparse squarings, one workgroup per row over a list of entries built
+  // before the squaring (by row, ascending columns; entry = (row << 16) |
+  // column, and its value):
+  //   row-sparse (*sq_mode 1): the input's entries other than the
+  //     (+)-identity, when there are at most rows_limit;
+  //   changed-entry (*sq_mode 2): the input's entries that differ from
+  //     `prev` (the previous squaring's input), when at most changed_limit.
+  //   *sq_mode 0: the tiled kernel squares.  sq_mode null: tiled only.
+  // A form whose *_auto flag is set is taken only when the cost model says
+  // it beats the other forms: cost_slab_tile per (tile, slab) pair the tiled
+  // kernel would compute (*work, counted before the squaring), cost_row_entry
+  // per listed entry, cost_changed_entry per changed entry, plus cost_list
+  // for either sparse form.
+  const float *prev;  // null: no changed-entry form for this squaring
+  int rows_limit, changed_limit;
+  int rows_auto, changed_auto;
+  float cost_slab_tile, cost_row_entry, cost_changed_entry, cost_list;
+  unsigned long long *work;
+  int *sq_mode;
+  int *rowptr;        // n + 1
+  int *chgcnt;        // n
+  uint32_t *lkj;
+  float *lval;
 };
 hipError_t launch_sched_square(int op, int cfg, const SchedArgs &a, hipStream_t s);
-// After a row-sparse squaring (nothing otherwise): occupancy maps and entry
-// count of its output.  With a == null: of `D` for the first squaring.
+// After a sparse squaring (nothing after a tiled one): occupancy maps of
+// its output.  With a == null: of `D` for the first squaring.
 hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, int n,
                                   int tiles_r, int tiles_c, int slabs, uint8_t *xocc,
-                                  uint8_t *yocc, int *nnz_out, const SchedArgs *a,
-                                  hipStream_t s);
+                                  uint8_t *yocc, const SchedArgs *a, hipStream_t s);
 hipError_t launch_fill_f32(float *p, float v, int64_t n, hipStream_t s);
 hipError_t launch_permute_copy(float *dst, int64_t ldd, const float *src, int64_t lds, int n,
                                const int *perm, int inverse, hipStream_t s);

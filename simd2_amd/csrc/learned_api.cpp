@@ -168,7 +168,7 @@ int simd2_converge_learned(int op, int n, float *X, int ldx, int do_epilogue, in
   // Schedule from the prediction.  Enqueueing a squaring too many costs a
   // launch (it returns at once); one too few costs a flag read per extra
   // squaring, hence the rounding up.
-  simd2_schedule sc{-1, 0, -1};
+  simd2_schedule sc{-1, 0, -1, -1};
   const float itr_pred = std::isfinite(host[0]) ? host[0] : 1.0f;
   sc.predicted_itr = std::max(1, std::min(max_itr, (int)std::ceil(itr_pred + 0.25f)));
   const float cls = std::isfinite(host[1]) ? host[1] : 2.0f;

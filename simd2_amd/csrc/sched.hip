@@ -46,9 +46,8 @@ __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) 
   // A squaring after one that changed nothing computes nothing: the result
   // stays where that squaring left it.
   if (a.conv_prev && *a.conv_prev == 0) return;
-  if (a.nnz_in && *a.nnz_in <= a.nnz_limit) return;  // squared row by row instead
+  if (a.sq_mode && *a.sq_mode != 0) return;  // squared row by row instead
   const int tid = tid_x();
-  if (a.nnz_out && bid_x() == 0 && tid == 0) *a.nnz_out = 0x7fffffff;  // not sparse any more
   int tr, tc;
   map_tile(bid_x(), a.tiles_r, a.tiles_c, tr, tc);
   if (a.rowchg_prev && !a.rowchg_prev[tr] && !a.colchg_prev[tc]) return;  // clean tile
@@ -154,135 +153,365 @@ __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) 
   }
 }
 
-// Row-sparse squaring, one workgroup per row i:
-//   D'[i][c] = [D[i][c] (+)] (+)_{k : D[i][k] != identity} D[i][k] (x) D[k][c].
-// The terms left out are identity (x) y = identity, so the result is the tiled
-// kernel's bit for bit; it costs one pass over row k per entry of row i, which
-// beats the tiled kernel while rows hold few entries (early squarings of
-// sparse graphs).  Row i's entries are compacted into LDS (wave ballots) in
-// passes of RL_CAP columns; each thread keeps RL_Q output columns in
-// registers, 8192 columns per chunk.
-constexpr int RL_Q = 32;
-constexpr int RL_CHUNK = EW * RL_Q;
-constexpr int RL_CAP = 2048;
+// Sparse squarings, one workgroup per output row i, over a list of entries
+// built before the squaring (by row, ascending columns):
+//   * row-sparse: every entry of the input other than the (+)-identity.
+//       D'[i][:] = [D[i][:] (+)] (+)_{(i,k) listed} D[i][k] (x) D[k][:]
+//     is the tiled result bit for bit: the terms left out are
+//     identity (x) y = identity.  It costs one pass over row k per entry.
+//   * changed-entry: the entries that changed in the previous squaring.  With
+//     the epilogue on, the input is D = D_prev (+) (+)_k D_prev[i][k] (x)
+//     D_prev[k][j], so a term D[i][k] (x) D[k][j] whose operands both kept
+//     their bits is a term of the previous squaring and already inside
+//     D[i][j]; (+) being min or max, taking it again changes no bit.  The
+//     remaining terms have (i,k) changed (a pass over row k, as above) or
+//     (k,j) changed (scattered into row i's accumulators in LDS).
+// Before each squaring: the (tile, slab) pairs the tiled kernel would compute
+// (its cost), per-row counts of both kinds of entries, one scan that picks
+// the form, then each row's entries compacted with wave ballots (entry =
+// (row << 16) | column, and the value).
+constexpr int SP_G = 8;                // 4-column groups per thread
+constexpr int SP_CH = EW * 4 * SP_G;   // columns per pass of a row
+constexpr int SCAN_T = 1024;
+constexpr int SP_B = 8;                // changed entries per thread per batch
+
+// (tile, slab) pairs the tiled kernel would compute in this squaring: dirty
+// tiles times the slabs occupied on both sides, plus two per dirty tile for
+// its epilogue.  One thread per tile.
+__global__ __launch_bounds__(EW) void tiled_work_kernel(SchedArgs a) {
+  if (a.conv_prev && *a.conv_prev == 0) return;
+  const int t = bid_x() * EW + tid_x();
+  unsigned long long w = 0;
+  if (t < a.tiles_r * a.tiles_c) {
+    const int tr = t / a.tiles_c, tc = t % a.tiles_c;
+    if (!a.rowchg_prev || a.rowchg_prev[tr] || a.colchg_prev[tc]) {
+      const uint8_t *xo = a.xocc_in + (int64_t)tr * a.slabs;
+      const uint8_t *yo = a.yocc_in + (int64_t)tc * a.slabs;
+      int cnt = 2;
+      for (int s = 0; s < a.slabs; ++s) cnt += xo[s] & yo[s];
+      w = (unsigned long long)cnt;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+  if (tid_x() % 64 == 0 && w) atomicAdd(a.work, w);
+}
+
+// Per row: entries other than the identity (rowptr[i]) and, with `prev`,
+// entries that differ from prev (chgcnt[i]).
+template <class Op>
+__global__ __launch_bounds__(EW) void list_count_kernel(SchedArgs a) {
+  __shared__ int s_wave[2][EW / 64];
+  if (a.conv_prev && *a.conv_prev == 0) return;
+  const int i = bid_x(), tid = tid_x(), lane = tid % 64, wave = tid / 64;
+  const uint32_t ident = __float_as_uint(Op::identity);
+  const int64_t r0 = (int64_t)i * a.ld;
+  int nz = 0, ch = 0;
+  // columns n .. ld-1 hold the identity in every buffer, so whole groups of
+  // four can be tested (ld is a multiple of 128)
+  for (int c = 4 * tid; c < a.n; c += 4 * EW) {
+    const vf4 x = *reinterpret_cast<const vf4 *>(a.cur + r0 + c);
+    nz += (__float_as_uint(x.x) != ident) + (__float_as_uint(x.y) != ident) +
+          (__float_as_uint(x.z) != ident) + (__float_as_uint(x.w) != ident);
+    if (a.prev) {
+      const vf4 y = *reinterpret_cast<const vf4 *>(a.prev + r0 + c);
+      ch += (__float_as_uint(x.x) != __float_as_uint(y.x)) +
+            (__float_as_uint(x.y) != __float_as_uint(y.y)) +
+            (__float_as_uint(x.z) != __float_as_uint(y.z)) +
+            (__float_as_uint(x.w) != __float_as_uint(y.w));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    nz += __shfl_xor(nz, o);
+    ch += __shfl_xor(ch, o);
+  }
+  if (lane == 0) {
+    s_wave[0][wave] = nz;
+    s_wave[1][wave] = ch;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    a.rowptr[i] = s_wave[0][0] + s_wave[0][1] + s_wave[0][2] + s_wave[0][3];
+    if (a.prev) a.chgcnt[i] = s_wave[1][0] + s_wave[1][1] + s_wave[1][2] + s_wave[1][3];
+  }
+}
+
+// Totals, the squaring's form, and the exclusive scan of the chosen counts
+// into rowptr (one workgroup).
+__global__ __launch_bounds__(SCAN_T) void list_scan_kernel(SchedArgs a) {
+  __shared__ long long s_nz[SCAN_T], s_ch[SCAN_T];
+  __shared__ int s_mode;
+  if (a.conv_prev && *a.conv_prev == 0) return;
+  const int tid = tid_x();
+  const int per = (a.n + SCAN_T - 1) / SCAN_T;
+  const int lo = min(a.n, tid * per), hi = min(a.n, lo + per);
+  long long nz = 0, ch = 0;
+  for (int r = lo; r < hi; ++r) {
+    nz += a.rowptr[r];
+    if (a.prev) ch += a.chgcnt[r];
+  }
+  s_nz[tid] = nz;
+  s_ch[tid] = ch;
+  __syncthreads();
+  for (int o = 1; o < SCAN_T; o <<= 1) {
+    const long long vn = tid >= o ? s_nz[tid - o] : 0, vc = tid >= o ? s_ch[tid - o] : 0;
+    __syncthreads();
+    s_nz[tid] += vn;
+    s_ch[tid] += vc;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const long long tn = s_nz[SCAN_T - 1], tc = s_ch[SCAN_T - 1];
+    // both limits are within the list's capacity
+    const bool rows_ok = a.rows_limit > 0 && tn <= a.rows_limit;
+    const bool ch_ok = a.prev && a.changed_limit > 0 && tc <= a.changed_limit;
+    int mode = 0;
+    if (rows_ok && !a.rows_auto) {
+      mode = 1;
+    } else if (ch_ok && !a.changed_auto) {
+      mode = 2;
+    } else {
+      const double c_rows = (double)tn * a.cost_row_entry + a.cost_list;
+      const double c_changed = (double)tc * a.cost_changed_entry + a.cost_list;
+      double best = (double)*a.work * a.cost_slab_tile;
+      if (rows_ok && c_rows < best) {
+        mode = 1;
+        best = c_rows;
+      }
+      if (ch_ok && c_changed < best) mode = 2;
+    }
+    *a.sq_mode = mode;
+    s_mode = mode;
+  }
+  __syncthreads();
+  const int mode = s_mode;
+  if (mode == 0) return;
+  const long long *sc = mode == 1 ? s_nz : s_ch;
+  const int *cnt = mode == 1 ? a.rowptr : a.chgcnt;
+  long long run = sc[tid] - (mode == 1 ? nz : ch);
+  for (int r = lo; r < hi; ++r) {
+    const int c = cnt[r];  // read before rowptr[r] is overwritten (mode 1)
+    a.rowptr[r] = (int)run;
+    run += c;
+  }
+  if (tid == SCAN_T - 1) a.rowptr[a.n] = (int)sc[SCAN_T - 1];
+}
 
 template <class Op>
-__global__ __launch_bounds__(EW) void sched_rows_kernel(SchedArgs a, int bm, int bn) {
-  __shared__ uint16_t lk[RL_CAP];
-  __shared__ float lx[RL_CAP];
-  __shared__ int s_cnt;
+__global__ __launch_bounds__(EW) void list_fill_kernel(SchedArgs a) {
   __shared__ int s_wave[EW / 64];
   if (a.conv_prev && *a.conv_prev == 0) return;
-  if (!a.nnz_in || *a.nnz_in > a.nnz_limit) return;  // squared tile by tile instead
+  const int mode = *a.sq_mode;
+  if (mode == 0) return;
   const int i = bid_x();
+  int at = a.rowptr[i];
+  const int end = a.rowptr[i + 1];
+  if (at == end) return;
   const int tid = tid_x(), lane = tid % 64, wave = tid / 64;
   const uint32_t ident = __float_as_uint(Op::identity);
-  const float *rowi = a.cur + (int64_t)i * a.ld;
-  float *outi = a.nxt + (int64_t)i * a.ld;
+  const int64_t r0 = (int64_t)i * a.ld;
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (int c0 = 0; c0 < a.n && at < end; c0 += 4 * EW) {
+    const int c = c0 + 4 * tid;
+    bool p[4] = {false, false, false, false};
+    if (c < a.n) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t x = __float_as_uint(a.cur[r0 + c + q]);
+        p[q] = mode == 1 ? x != ident : x != __float_as_uint(a.prev[r0 + c + q]);
+      }
+    }
+    int before = 0, wave_total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint64_t m = __ballot(p[q]);
+      before += __popcll(m & below);
+      wave_total += __popcll(m);
+    }
+    if (lane == 0) s_wave[wave] = wave_total;
+    __syncthreads();
+    int off = at + before;
+    for (int w = 0; w < wave; ++w) off += s_wave[w];
+    const int total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    __syncthreads();  // s_wave is rewritten by the next group
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (p[q] && off < end) {
+        a.lkj[off] = ((uint32_t)i << 16) | (uint32_t)(c + q);
+        a.lval[off] = a.cur[r0 + c + q];
+        ++off;
+      }
+    at += total;
+  }
+}
+
+template <class Op>
+__device__ __forceinline__ void lds_combine(float *p, float v) {
+  if constexpr (Op::identity > 0.0f)
+    __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <class Op>
+__device__ __forceinline__ vf4 relax4(vf4 acc, float x0, vf4 y0, float x1, vf4 y1) {
+  acc.x = Op::add3(acc.x, Op::mul(x0, y0.x), Op::mul(x1, y1.x));
+  acc.y = Op::add3(acc.y, Op::mul(x0, y0.y), Op::mul(x1, y1.y));
+  acc.z = Op::add3(acc.z, Op::mul(x0, y0.z), Op::mul(x1, y1.z));
+  acc.w = Op::add3(acc.w, Op::mul(x0, y0.w), Op::mul(x1, y1.w));
+  return acc;
+}
+
+// One output row per workgroup; CHANGED selects the changed-entry form.
+template <class Op, bool CHANGED>
+__global__ __launch_bounds__(EW) void sparse_square_kernel(SchedArgs a, int bm, int bn) {
+  __shared__ float lds_acc[CHANGED ? SP_CH : 1];
+  __shared__ float lds_x[CHANGED ? SP_CH : 1];
+  if (a.conv_prev && *a.conv_prev == 0) return;
+  if (*a.sq_mode != (CHANGED ? 2 : 1)) return;
+  const int i = bid_x(), tid = tid_x(), lane = tid % 64;
+  const uint32_t ident = __float_as_uint(Op::identity);
+  const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+  const int64_t r0 = (int64_t)i * a.ld;
   bool chg_bits = false, chg_test = false;
-  long long entries = 0;
-  for (int c0 = 0; c0 < a.n; c0 += RL_CHUNK) {
-    float acc[RL_Q];
-#pragma unroll
-    for (int q = 0; q < RL_Q; ++q) acc[q] = Op::identity;
-    for (int k0 = 0; k0 < a.n; k0 += RL_CAP) {
-      if (tid == 0) s_cnt = 0;
-      __syncthreads();
-      const int k1 = min(a.n, k0 + RL_CAP);
-      for (int kb = k0; kb < k1; kb += EW) {
-        const int k = kb + tid;
-        const float x = k < k1 ? rowi[k] : Op::identity;
-        const bool p = k < k1 && __float_as_uint(x) != ident;
-        const uint64_t m = __ballot(p);
-        if (lane == 0) s_wave[wave] = __popcll(m);
+  for (int jc = 0; jc < a.n; jc += SP_CH) {
+    if constexpr (CHANGED) {
+      // terms with (k, j) changed: lds_acc[j] (+)= D[i][k] (x) D[k][j]
+      for (int t = tid; t < SP_CH; t += EW) lds_acc[t] = Op::identity;
+      for (int kc = 0; kc < a.n; kc += SP_CH) {
+        const int lo = a.rowptr[kc], hi = a.rowptr[min(a.n, kc + SP_CH)];
+        if (lo == hi) continue;
+        __syncthreads();  // lds_x readers of the previous chunk are done
+        for (int t = tid; t < SP_CH; t += EW)
+          lds_x[t] = kc + t < a.n ? a.cur[r0 + kc + t] : Op::identity;
         __syncthreads();
-        int off = s_cnt;
-        for (int w = 0; w < wave; ++w) off += s_wave[w];
-        if (p) {
-          const int at = off + __popcll(m & ((1ull << lane) - 1ull));
-          lk[at] = (uint16_t)k;
-          lx[at] = x;
+        // SP_B entries per thread are loaded before any is used, so that
+        // many list reads are in flight
+        for (int base = lo; base < hi; base += EW * SP_B) {
+          uint32_t u[SP_B];
+          float v[SP_B];
+#pragma unroll
+          for (int q = 0; q < SP_B; ++q) {
+            const int e = base + EW * q + tid;
+            u[q] = e < hi ? a.lkj[e] : 0xffffffffu;  // column 0xffff: never in range
+            v[q] = e < hi ? a.lval[e] : Op::identity;
+          }
+#pragma unroll
+          for (int q = 0; q < SP_B; ++q) {
+            const int j = (int)(u[q] & 0xffffu) - jc;
+            if ((unsigned)j < (unsigned)SP_CH && (int)(u[q] & 0xffffu) < a.n) {
+              const float x = lds_x[(int)(u[q] >> 16) - kc];
+              if (__float_as_uint(x) != ident) lds_combine<Op>(&lds_acc[j], Op::mul(x, v[q]));
+            }
+          }
         }
-        __syncthreads();
-        if (tid == 0) s_cnt += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
-        __syncthreads();
       }
-      const int cnt = s_cnt;
-      entries += cnt;
-      for (int e = 0; e < cnt; ++e) {
-        const float x = lx[e];
-        const float *rowk = a.cur + (int64_t)lk[e] * a.ld + c0 + tid;
-#pragma unroll
-        for (int q = 0; q < RL_Q; ++q)
-          if (c0 + tid + EW * q < a.ld) acc[q] = Op::add(acc[q], Op::mul(x, rowk[EW * q]));
-      }
-      __syncthreads();  // the list is rewritten by the next pass
+      __syncthreads();
     }
+    // terms with (i, k) listed: a pass over row k per entry, two at a time
+    vf4 acc[SP_G];
 #pragma unroll
-    for (int q = 0; q < RL_Q; ++q) {
-      const int c = c0 + tid + EW * q;
-      bool col_chg = false;
+    for (int g = 0; g < SP_G; ++g) acc[g] = vf4{Op::identity, Op::identity, Op::identity, Op::identity};
+    const int cbase = jc + 4 * tid;
+    int e = e0;
+    for (; e + 1 < e1; e += 2) {
+      const float x0 = a.lval[e], x1 = a.lval[e + 1];
+      const float *y0p = a.cur + (int64_t)(a.lkj[e] & 0xffffu) * a.ld + cbase;
+      const float *y1p = a.cur + (int64_t)(a.lkj[e + 1] & 0xffffu) * a.ld + cbase;
+      vf4 y0[SP_G], y1[SP_G];
+#pragma unroll
+      for (int g = 0; g < SP_G; ++g)
+        if (cbase + EW * 4 * g < a.n) {
+          y0[g] = *reinterpret_cast<const vf4 *>(y0p + EW * 4 * g);
+          y1[g] = *reinterpret_cast<const vf4 *>(y1p + EW * 4 * g);
+        }
+#pragma unroll
+      for (int g = 0; g < SP_G; ++g)
+        if (cbase + EW * 4 * g < a.n) acc[g] = relax4<Op>(acc[g], x0, y0[g], x1, y1[g]);
+    }
+    if (e < e1) {
+      const float x0 = a.lval[e];
+      const float *y0p = a.cur + (int64_t)(a.lkj[e] & 0xffffu) * a.ld + cbase;
+#pragma unroll
+      for (int g = 0; g < SP_G; ++g)
+        if (cbase + EW * 4 * g < a.n) {
+          const vf4 y = *reinterpret_cast<const vf4 *>(y0p + EW * 4 * g);
+          acc[g] = relax4<Op>(acc[g], x0, y, x0, y);
+        }
+    }
+    // epilogue: combine, compare with the input, store
+#pragma unroll
+    for (int g = 0; g < SP_G; ++g) {
+      const int c = cbase + EW * 4 * g;
       if (c < a.n) {
-        const float p = rowi[c];
-        float v = acc[q];
-        if (a.epi) v = Op::add(v, p);
-        if (a.mode == CHECK_APLP && v > -1.0f && v < 1.0f) v = -FLT_MAX;
-        col_chg = __float_as_uint(v) != __float_as_uint(p);
-        chg_bits |= col_chg;
-        chg_test |= differs(v, p, a.mode);
-        outi[c] = v;
+        const vf4 p = *reinterpret_cast<const vf4 *>(a.cur + r0 + c);
+        vf4 v = acc[g];
+        bool col = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          float t = v[q];
+          if constexpr (CHANGED) t = Op::add(t, lds_acc[c - jc + q]);
+          if (a.epi) t = Op::add(t, p[q]);
+          if (!CHANGED && a.mode == CHECK_APLP && t > -1.0f && t < 1.0f) t = -FLT_MAX;
+          v[q] = t;
+          if (c + q < a.n) {
+            col |= __float_as_uint(t) != __float_as_uint(p[q]);
+            chg_test |= differs(t, p[q], a.mode);
+          }
+        }
+        chg_bits |= col;
+        if (col) a.colchg[c / bn] = 1;  // c .. c+3 lie in one column block
+        float *o = a.nxt + r0 + c;
+        if (c + 4 <= a.n) {
+          *reinterpret_cast<vf4 *>(o) = v;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (c + q < a.n) o[q] = v[q];
+        }
       }
-      // a wave covers 64 consecutive columns, inside one column block
-      if (__any(col_chg) && lane == 0) a.colchg[(c0 + EW * q + 64 * wave) / bn] = 1;
     }
+    if constexpr (CHANGED) __syncthreads();  // lds_acc is reset by the next pass
   }
   if (__any(chg_bits) && lane == 0) a.rowchg[i / bm] = 1;
   if (__any(chg_test) && lane == 0) *a.conv = 1;
   if (tid == 0 && a.stats) {
-    atomicAdd(a.stats + 2, 1ull);                               // rows squared row-sparse
-    atomicAdd(a.stats + 3, (unsigned long long)entries);        // row entries visited
+    atomicAdd(a.stats + (CHANGED ? 4 : 2), 1ull);
+    atomicAdd(a.stats + (CHANGED ? 5 : 3), (unsigned long long)(e1 - e0));
   }
 }
 
 // Occupancy maps of a matrix, one workgroup per bm x bn tile (the maps the
 // tiled squaring's epilogue writes), and its count of entries other than the
-// identity.  Used for the first squaring's input and after row-sparse
+// identity.  Used for the first squaring's input and after sparse
 // squarings; after a tiled squaring (or once converged) it does nothing.
 __global__ __launch_bounds__(EW) void occupancy_kernel(const float *D, int64_t ld, int n, int bm,
                                                        int bn, int tiles_c, int slabs,
                                                        uint32_t ident, uint8_t *xocc,
-                                                       uint8_t *yocc, int *nnz_out,
-                                                       const int *conv_prev, const int *nnz_in,
-                                                       int nnz_limit) {
+                                                       uint8_t *yocc, const int *conv_prev,
+                                                       const int *sq_mode) {
   __shared__ unsigned s_bits[2];
-  __shared__ int s_count;
   if (conv_prev && *conv_prev == 0) return;
-  if (nnz_in && *nnz_in > nnz_limit) return;
+  if (sq_mode && *sq_mode == 0) return;
   const int tid = tid_x();
   const int tr = bid_x() / tiles_c, tc = bid_x() % tiles_c;
   const int row0 = tr * bm, col0 = tc * bn;
   if (tid == 0) {
     s_bits[0] = 0;
     s_bits[1] = 0;
-    s_count = 0;
   }
   __syncthreads();
   unsigned xb = 0, yb = 0;
-  int count = 0;
   for (int e = tid; e < bm * bn; e += EW) {
     const int rl = e / bn, cl = e % bn;
     const int r = row0 + rl, c = col0 + cl;
     if (r < n && c < n && __float_as_uint(D[(int64_t)r * ld + c]) != ident) {
       xb |= 1u << (cl / 16);
       yb |= 1u << (rl / 16);
-      ++count;
     }
   }
   if (xb) atomicOr(&s_bits[0], xb);
   if (yb) atomicOr(&s_bits[1], yb);
-  if (count) atomicAdd(&s_count, count);
   __syncthreads();
-  if (tid == 0 && nnz_out && s_count) atomicAdd(nnz_out, s_count);
   const int xs = bn / 16, ys = bm / 16;
   if (tid < xs) {
     const int s = col0 / 16 + tid;
@@ -318,9 +547,16 @@ int grid_for(int64_t n) { return (int)std::min<int64_t>((n + EW - 1) / EW, 4096)
 template <class Op>
 hipError_t sched_op(int cfg, const SchedArgs &a, hipStream_t s) {
   const dim3 grid(a.tiles_r * a.tiles_c), block(256);
-  if (a.nnz_in) {  // the row-sparse form; exactly one of the two kernels works
+  if (a.sq_mode) {  // the list, then whichever of the three forms it chose works
     const ValuCfgInfo c = valu_cfg_info(cfg);
-    sched_rows_kernel<Op><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
+    const int tiles = a.tiles_r * a.tiles_c;
+    tiled_work_kernel<<<dim3((tiles + EW - 1) / EW), dim3(EW), 0, s>>>(a);
+    list_count_kernel<Op><<<dim3(a.n), dim3(EW), 0, s>>>(a);
+    list_scan_kernel<<<dim3(1), dim3(SCAN_T), 0, s>>>(a);
+    list_fill_kernel<Op><<<dim3(a.n), dim3(EW), 0, s>>>(a);
+    if (a.rows_limit > 0)
+      sparse_square_kernel<Op, false><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
+    if (a.prev) sparse_square_kernel<Op, true><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
   }
   if (cfg == CFG_S64)
     sched_kernel<Op, CfgS><<<grid, block, 0, s>>>(a);
@@ -345,15 +581,14 @@ hipError_t launch_sched_square(int op, int cfg, const SchedArgs &a, hipStream_t 
 
 hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, int n,
                                   int tiles_r, int tiles_c, int slabs, uint8_t *xocc,
-                                  uint8_t *yocc, int *nnz_out, const SchedArgs *a,
-                                  hipStream_t s) {
+                                  uint8_t *yocc, const SchedArgs *a, hipStream_t s) {
   const ValuCfgInfo c = valu_cfg_info(cfg);
   const float ident = (op == 0 || op == 6) ? SIMD2_INF : -SIMD2_INF;
   uint32_t bits;
   std::memcpy(&bits, &ident, 4);
   occupancy_kernel<<<dim3(tiles_r * tiles_c), dim3(EW), 0, s>>>(
-      D, ld, n, c.bm, c.bn, tiles_c, slabs, bits, xocc, yocc, nnz_out,
-      a ? a->conv_prev : nullptr, a ? a->nnz_in : nullptr, a ? a->nnz_limit : 0);
+      D, ld, n, c.bm, c.bn, tiles_c, slabs, bits, xocc, yocc, a ? a->conv_prev : nullptr,
+      a ? a->sq_mode : nullptr);
   return hipGetLastError();
 }
 

@@ -28,10 +28,31 @@ bool sched_op_supported(int op) {
          op == SIMD2_MIN_MAX;
 }
 
+// Entries per row (on average) the list of a sparse squaring can hold.
+int list_per_row(int n) { return std::min(n, 1024); }
+
+// The automatic sparse forms start at this size: below it the list's kernel
+// launches cost about as much as a whole tiled squaring.
+constexpr int SPARSE_AUTO_MIN_N = 2048;
+
+// Cost model of one squaring, in picoseconds, measured on MI355X at
+// n = 8192 with these kernels (every form is exact; the model only picks the
+// fastest):
+//   tiled: per (i, j, k) term, 13.3 ms / 8192^3 for min_plus and max_plus
+//     (add + min3/max3), 22.6 ms / 8192^3 for max_min and min_max;
+//   row-sparse: per (listed entry, column), from the slope of the squaring
+//     time over 4 .. 512 entries per row (0.034 ms per entry per row);
+//   changed-entry: per (changed entry, n), a row pass plus a scatter into
+//     every row's accumulators (ER graphs with 60 and 80 changed per row);
+//   either sparse form: two passes over the matrix (list fill, occupancy).
+constexpr double PS_TERM_ADD = 0.0242, PS_TERM_MINMAX = 0.0411;
+constexpr double PS_ROW_ENTRY_COL = 0.507, PS_CHANGED_ENTRY_N = 1.8, PS_LIST_ELEM = 1.3;
+
 // Workspace carve-up (every piece 256-byte aligned).
 struct Layout {
   int P, kpad, tiles_r, tiles_c, slabs;
-  size_t buf[2], xocc[2], yocc[2], rowchg, colchg, conv, nnz, stats, total;
+  size_t buf[2], xocc[2], yocc[2], rowchg, colchg, conv, work, mode, rowptr, chgcnt, lkj, lval,
+      stats, total;
 };
 
 bool layout(int n, int cfg, int max_itr, Layout *L) {
@@ -60,8 +81,14 @@ bool layout(int n, int cfg, int max_itr, Layout *L) {
   L->rowchg = take(rounds * L->tiles_r * sizeof(int));
   L->colchg = take(rounds * L->tiles_c * sizeof(int));
   L->conv = take(rounds * sizeof(int));
-  L->nnz = take((rounds + 1) * sizeof(int));
-  L->stats = take(4 * sizeof(unsigned long long));
+  L->work = take(rounds * sizeof(unsigned long long));
+  L->mode = take(rounds * sizeof(int));
+  L->rowptr = take(((size_t)n + 1) * sizeof(int));
+  L->chgcnt = take((size_t)n * sizeof(int));
+  const size_t cap = (size_t)list_per_row(n) * n;
+  L->lkj = take(cap * sizeof(uint32_t));
+  L->lval = take(cap * sizeof(float));
+  L->stats = take(6 * sizeof(unsigned long long));
   L->total = off;
   return true;
 }
@@ -88,12 +115,23 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   if (n == 0) return SIMD2_SUCCESS;
   if (ldx < n || n > 16 * SCHED_MAX_SLABS) return SIMD2_ERROR_INVALID_PROBLEM;
   if (!X || !workspace) return SIMD2_ERROR_NULL_POINTER;
-  simd2_schedule sc = schedule ? *schedule : simd2_schedule{-1, 0, -1};
+  simd2_schedule sc = schedule ? *schedule : simd2_schedule{-1, 0, -1, -1};
   if (sc.tile_config >= CFG_COUNT || sc.predicted_itr < 0) return SIMD2_ERROR_INVALID_PROBLEM;
   const int cfg = sc.tile_config >= 0 ? sc.tile_config : valu_pick_cfg(op, n, n, n);
-  // row-sparse squarings up to `row_sparse` entries per row on average
-  const int per_row = sc.row_sparse < 0 ? 128 : std::min(sc.row_sparse, n);
+  // Sparse forms: a fixed limit (entries per row on average) or, with -1,
+  // up to the list's capacity where the cost model favours them.  The
+  // changed-entry form needs the epilogue and no APLP mapping: it rests on
+  // D = D_prev (+) D_prev (x) D_prev.
+  const bool rows_auto = sc.row_sparse < 0, changed_auto = sc.changed_sparse < 0;
+  const int per_row = rows_auto ? (n >= SPARSE_AUTO_MIN_N ? list_per_row(n) : 0)
+                                : std::min(sc.row_sparse, list_per_row(n));
+  const int changed_per_row = changed_auto ? (n >= SPARSE_AUTO_MIN_N ? list_per_row(n) : 0)
+                                           : std::min(sc.changed_sparse, list_per_row(n));
   const bool rows_on = per_row > 0;
+  const bool changed_on = changed_per_row > 0 && do_epilogue && check_mode != CHECK_APLP;
+  const ValuCfgInfo ci = valu_cfg_info(cfg);
+  const bool addmul = op == SIMD2_MIN_PLUS || op == SIMD2_MAX_PLUS;
+  const double ps_term = addmul ? PS_TERM_ADD : PS_TERM_MINMAX;
   Layout L;
   if (!layout(n, cfg, max_itr, &L)) return SIMD2_ERROR_INVALID_PROBLEM;
   if (workspace_bytes < L.total) return SIMD2_ERROR_WORKSPACE;
@@ -108,7 +146,12 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   int *rowchg = reinterpret_cast<int *>(ws + L.rowchg);
   int *colchg = reinterpret_cast<int *>(ws + L.colchg);
   int *conv = reinterpret_cast<int *>(ws + L.conv);
-  int *nnz = reinterpret_cast<int *>(ws + L.nnz);
+  auto *work = reinterpret_cast<unsigned long long *>(ws + L.work);
+  int *mode = reinterpret_cast<int *>(ws + L.mode);
+  int *rowptr = reinterpret_cast<int *>(ws + L.rowptr);
+  int *chgcnt = reinterpret_cast<int *>(ws + L.chgcnt);
+  uint32_t *lkj = reinterpret_cast<uint32_t *>(ws + L.lkj);
+  float *lval = reinterpret_cast<float *>(ws + L.lval);
   unsigned long long *stats = reinterpret_cast<unsigned long long *>(ws + L.stats);
   const float inf = __builtin_inff();
   const float ident = (op == SIMD2_MIN_PLUS || op == SIMD2_MIN_MAX) ? inf : -inf;
@@ -121,11 +164,13 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   if (e == hipSuccess) e = launch_fill_i32(rowchg, 0, (int64_t)(max_itr + 1) * L.tiles_r, stream);
   if (e == hipSuccess) e = launch_fill_i32(colchg, 0, (int64_t)(max_itr + 1) * L.tiles_c, stream);
   if (e == hipSuccess) e = launch_fill_i32(conv, 0, max_itr + 1, stream);
-  if (e == hipSuccess) e = launch_fill_i32(nnz, 0, max_itr + 2, stream);
-  if (e == hipSuccess) e = launch_fill_i32(reinterpret_cast<int *>(stats), 0, 8, stream);
+  if (e == hipSuccess)
+    e = launch_fill_i32(reinterpret_cast<int *>(work), 0, 2 * (int64_t)(max_itr + 1), stream);
+  if (e == hipSuccess) e = launch_fill_i32(mode, 0, max_itr + 1, stream);
+  if (e == hipSuccess) e = launch_fill_i32(reinterpret_cast<int *>(stats), 0, 12, stream);
   if (e == hipSuccess)
     e = launch_sched_occupancy(op, cfg, buf[0], L.P, n, L.tiles_r, L.tiles_c, L.slabs, xocc[0],
-                               yocc[0], nnz, nullptr, stream);
+                               yocc[0], nullptr, stream);
   if (e != hipSuccess) return hip_status(e);
 
   int t = 0, T = -1, syncs = 0;
@@ -157,13 +202,27 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
       a.conv_prev = s > 0 ? conv + s - 1 : nullptr;
       a.conv = conv + s;
       a.stats = stats;
-      a.nnz_in = rows_on ? nnz + s : nullptr;
-      a.nnz_out = rows_on ? nnz + s + 1 : nullptr;
-      a.nnz_limit = per_row * n;
+      // the previous squaring's input is the buffer this one overwrites; it
+      // is read (to list the changed entries) before anything is written
+      a.prev = changed_on && s > 0 ? a.nxt : nullptr;
+      a.rows_limit = rows_on ? per_row * n : 0;
+      a.changed_limit = a.prev ? changed_per_row * n : 0;
+      a.rows_auto = rows_auto ? 1 : 0;
+      a.changed_auto = changed_auto ? 1 : 0;
+      a.cost_slab_tile = (float)(16.0 * ci.bm * ci.bn * ps_term);
+      a.cost_list = (float)((double)n * n * PS_LIST_ELEM);
+      a.cost_row_entry = (float)(n * PS_ROW_ENTRY_COL);
+      a.cost_changed_entry = (float)(n * PS_CHANGED_ENTRY_N);
+      a.work = work + s;
+      a.sq_mode = rows_on || a.prev ? mode + s : nullptr;
+      a.rowptr = rowptr;
+      a.chgcnt = chgcnt;
+      a.lkj = lkj;
+      a.lval = lval;
       e = launch_sched_square(op, cfg, a, stream);
-      if (e == hipSuccess && rows_on)  // maps and count after a row-sparse squaring
+      if (e == hipSuccess && a.sq_mode)  // maps after a sparse squaring
         e = launch_sched_occupancy(op, cfg, a.nxt, L.P, n, L.tiles_r, L.tiles_c, L.slabs,
-                                   a.xocc_out, a.yocc_out, a.nnz_out, &a, stream);
+                                   a.xocc_out, a.yocc_out, &a, stream);
     }
     if (e != hipSuccess) return hip_status(e);
     flags.resize(m);
@@ -183,7 +242,7 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   const int done = T >= 0 ? T + 1 : t;  // squarings whose result counts
   float *result = buf[done & 1];
   e = launch_permute_copy(X, ldx, result, L.P, n, perm, 1, stream);
-  unsigned long long st[4] = {0, 0, 0, 0};
+  unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
   if (e == hipSuccess && report)
     e = hipMemcpyAsync(st, stats, sizeof(st), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -198,6 +257,8 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
     report->slab_tiles_computed = st[1];
     report->rows_sparse = st[2];
     report->row_entries = st[3];
+    report->changed_rows = st[4];
+    report->changed_entries = st[5];
   }
   return T >= 0 ? SIMD2_SUCCESS : SIMD2_ERROR_NOT_CONVERGED;
 }

@@ -8,7 +8,7 @@ from . import CHECK, OP, OPS, Simd2Error, SUCCESS, ERROR_NOT_CONVERGED, _ptr, _s
 
 class Schedule(ctypes.Structure):
     _fields_ = [("tile_config", ctypes.c_int), ("predicted_itr", ctypes.c_int),
-                ("row_sparse", ctypes.c_int)]
+                ("row_sparse", ctypes.c_int), ("changed_sparse", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
@@ -16,7 +16,8 @@ class Report(ctypes.Structure):
                 ("tiles_total", ctypes.c_ulonglong), ("tiles_computed", ctypes.c_ulonglong),
                 ("slab_tiles_total", ctypes.c_ulonglong),
                 ("slab_tiles_computed", ctypes.c_ulonglong),
-                ("rows_sparse", ctypes.c_ulonglong), ("row_entries", ctypes.c_ulonglong)]
+                ("rows_sparse", ctypes.c_ulonglong), ("row_entries", ctypes.c_ulonglong),
+                ("changed_rows", ctypes.c_ulonglong), ("changed_entries", ctypes.c_ulonglong)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -42,7 +43,8 @@ def workspace_bytes(n, tile_config=-1, max_itr=64):
 
 
 def converge_scheduled(op, X, epilogue=True, check="threshold", max_itr=64, tile_config=-1,
-                       predicted_itr=0, perm=None, workspace=None, stream=None, row_sparse=-1):
+                       predicted_itr=0, perm=None, workspace=None, stream=None, row_sparse=-1,
+                       changed_sparse=-1):
     """Scheduled converge loop on the n x n float32 device tensor X, in place.
     Returns (X, num_itr, status, report dict)."""
     torch = _torch()
@@ -56,7 +58,7 @@ def converge_scheduled(op, X, epilogue=True, check="threshold", max_itr=64, tile
         perm = perm.to(device=X.device, dtype=torch.int32).contiguous()
     itr = ctypes.c_int(0)
     rep = Report()
-    sc = Schedule(int(tile_config), int(predicted_itr), int(row_sparse))
+    sc = Schedule(int(tile_config), int(predicted_itr), int(row_sparse), int(changed_sparse))
     st = L.simd2_converge_scheduled(OP[opname], n, _ptr(X), X.stride(0), int(bool(epilogue)),
                                     CHECK[check], max_itr, ctypes.byref(sc), _ptr(perm),
                                     _ptr(workspace), workspace.numel(), _stream(stream),

@@ -48,13 +48,17 @@ def test_matches_plain_loop(gpu, kind, n):
             X = dag(torch, X)
         Y, itr, st = plain(torch, op, X, epi, check)
         assert st == simd2_amd.SUCCESS
-        # (tile shape, squarings enqueued ahead, row-sparse limit): the default
-        # limit, tiles only, and every squaring row by row
-        for cfg, pred, rs in ((-1, 0, -1), (0, 3, 0), (1, 64, n), (2, itr, -1)):
-            print(f"{kind} n={n} {op} cfg={cfg} pred={pred} row_sparse={rs}", flush=True)
+        # (tile shape, squarings enqueued ahead, row-sparse limit, changed-entry
+        # limit): the defaults, tiles only, every squaring row by row, every
+        # squaring after the first over the changed entries
+        changed_ok = epi and check != "aplp"
+        for cfg, pred, rs, cs in ((-1, 0, -1, -1), (0, 3, 0, 0), (1, 64, n, -1),
+                                  (2, itr, 0, n)):
+            print(f"{kind} n={n} {op} cfg={cfg} pred={pred} row_sparse={rs} "
+                  f"changed_sparse={cs}", flush=True)
             Z, itr2, st2, rep = converge_scheduled(op, X.clone(), epilogue=epi, check=check,
                                                    tile_config=cfg, predicted_itr=pred,
-                                                   row_sparse=rs)
+                                                   row_sparse=rs, changed_sparse=cs)
             assert st2 == simd2_amd.SUCCESS and itr2 == itr, (itr2, itr)
             assert torch.equal(Z.view(torch.int32), Y.view(torch.int32))
             assert rep["tiles_computed"] <= rep["tiles_total"]
@@ -62,6 +66,13 @@ def test_matches_plain_loop(gpu, kind, n):
                 assert rep["rows_sparse"] == n * itr and rep["tiles_computed"] == 0
             if rs == 0:
                 assert rep["rows_sparse"] == 0
+            if cs == 0 or not changed_ok:
+                assert rep["changed_rows"] == 0
+            if cs == n and rs == 0:
+                want = n * (itr - 1) if changed_ok else 0
+                assert rep["changed_rows"] == want, rep
+                if changed_ok:  # only the first squaring went tile by tile
+                    assert rep["tiles_computed"] <= rep["tiles_total"] // itr
             if pred >= itr:
                 assert rep["host_syncs"] == 1
             if pred == 0:
@@ -72,13 +83,22 @@ def test_skipping_happens_on_banded_graphs(gpu):
     torch = gpu
     X = grid_family(torch, "min_plus", 2048, 5)
     Y, itr, _ = plain(torch, "min_plus", X, True, "threshold")
-    Z, itr2, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=1, row_sparse=0)
+    Z, itr2, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=1, row_sparse=0,
+                                         changed_sparse=0)
     assert itr2 == itr and torch.equal(Z, Y)
     # the early squarings of a banded matrix touch few slabs
     assert rep["slab_tiles_computed"] < 0.6 * rep["slab_tiles_total"], rep
-    # with row-sparse squarings the first ones go row by row
+    # the automatic choice among tiled, row-sparse and changed-entry forms
     Z, itr3, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=1)
-    assert itr3 == itr and torch.equal(Z, Y) and rep["rows_sparse"] >= 2048, rep
+    assert itr3 == itr and torch.equal(Z, Y), rep
+    # the sparse forms wherever the list holds them (up to 1024 entries per
+    # row): the early squarings row by row, the late ones over the changed
+    # entries
+    for rs, cs, key, least in ((2048, 0, "rows_sparse", 3), (0, 2048, "changed_rows", 2)):
+        Z, itr3, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=1,
+                                             row_sparse=rs, changed_sparse=cs)
+        assert itr3 == itr and torch.equal(Z, Y), rep
+        assert rep[key] % 2048 == 0 and rep[key] >= least * 2048, rep
 
 
 def test_vertex_order_does_not_change_result(gpu):
