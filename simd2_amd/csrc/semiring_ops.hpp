@@ -43,20 +43,12 @@ __device__ __forceinline__ float vadd(float a, float b) {
   asm("v_add_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
   return r;
 }
-__device__ __forceinline__ float vmul(float a, float b) {
-  float r;
-  asm("v_mul_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-  return r;
-}
-__device__ __forceinline__ float vsub(float a, float b) {
-  float r;
-  asm("v_sub_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-  return r;
-}
+// c + a * b with one rounding, as the two-address v_fmac_f32: the
+// accumulator needs no second register (89 instead of 103 VGPRs in the
+// 128x64 minus_square tile; same rate as v_fma_f32).
 __device__ __forceinline__ float vfma(float a, float b, float c) {
-  float r;
-  asm("v_fma_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-  return r;
+  asm("v_fmac_f32 %0, %1, %2" : "+v"(c) : "v"(a), "v"(b));
+  return c;
 }
 
 struct MinPlus {
@@ -74,7 +66,9 @@ struct PlusMul {  // VALU form; the default plus_mul path is the MFMA kernel
   static constexpr int code = 1;
   static constexpr float identity = 0.0f;
   static constexpr float pad_x = 0.0f, pad_y = 0.0f;
-  __device__ __forceinline__ static float mul(float a, float b) { return vmul(a, b); }
+  // plain C product (an asm v_mul_f32 feeding the asm adds costs an s_nop
+  // per pair); the asm adds keep it from being contracted into an fma
+  __device__ __forceinline__ static float mul(float a, float b) { return a * b; }
   __device__ __forceinline__ static float add(float a, float b) { return a + b; }
   // Sequential in k with one rounding per op, as the oracle accumulates.
   __device__ __forceinline__ static float add3(float acc, float x, float y) {
@@ -150,10 +144,12 @@ struct MinusSquare {  // direct form: sum_k (a - b)^2, the square fused into the
   __device__ __forceinline__ static float add3(float acc, float x, float y) {
     return (acc + x) + y;
   }
-  // Fused form used by the tiled kernel: acc + (a-b)^2 as one v_fma_f32.
+  // Fused form used by the tiled kernel: acc + (a-b)^2 as one v_fma_f32.  The
+  // difference is plain C: an asm v_sub_f32 feeding the asm v_fma_f32 makes
+  // the compiler put an s_nop between every such pair.
   static constexpr bool fused = true;
   __device__ __forceinline__ static float fma_step(float acc, float a, float b) {
-    const float d = vsub(a, b);
+    const float d = a - b;
     return vfma(d, d, acc);
   }
 };
