@@ -117,8 +117,13 @@ __global__ __launch_bounds__(256) void pack_adj_kernel(const float *X, int64_t l
 // (-1: unreached).  Every level adds at least one vertex or ends the search,
 // so it stops within n levels.  s_cnt: 3 ints of LDS.
 __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *nxt, int *level,
-                    int *res, uint32_t *visited, int *s_cnt) {
+                    int *res, uint32_t *visited, uint32_t *red, int *s_cnt) {
   const int t = tid_x();
+  // Short adjacency rows (nw <= 128 words, n <= 4096) would leave most
+  // threads idle: the threads form G groups of wpg words, each group expands
+  // its share of the frontier, and the groups' words are OR-ed through LDS.
+  const int wpg = nw <= 64 ? 64 : (nw <= 128 ? 128 : BFS_T);
+  const int G = BFS_T / wpg, g = t / wpg, tw = t % wpg;
   for (int w = t; w < nw; w += BFS_T) visited[w] = 0;
   if (level)
     for (int v = t; v < n; v += BFS_T) level[v] = v == src ? 0 : -1;
@@ -135,14 +140,18 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
   for (;;) {
     const int cnt = s_cnt[0];
     uint32_t acc[BFS_W] = {0u, 0u, 0u, 0u};
-    // BFS_U frontier vertices at a time, so that many row loads are in flight
-    for (int base = 0; base < cnt; base += BFS_U) {
+    // BFS_U frontier vertices per group at a time, so that many row loads
+    // are in flight
+    for (int base = 0; base < cnt; base += BFS_U * G) {
       int u[BFS_U];
 #pragma unroll
-      for (int k = 0; k < BFS_U; ++k) u[k] = base + k < cnt ? cur[base + k] : -1;
+      for (int k = 0; k < BFS_U; ++k) {
+        const int at = base + g + G * k;
+        u[k] = at < cnt ? cur[at] : -1;
+      }
 #pragma unroll
       for (int q = 0; q < BFS_W; ++q) {
-        const int w = t + BFS_T * q;
+        const int w = tw + BFS_T * q;
         if (w < nw) {
 #pragma unroll
           for (int k = 0; k < BFS_U; ++k)
@@ -150,10 +159,16 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
         }
       }
     }
+    if (G > 1) {  // nw <= wpg < BFS_T: one word per thread
+      red[t] = acc[0];
+      __syncthreads();
+      if (g == 0)
+        for (int h = 1; h < G; ++h) acc[0] |= red[h * wpg + tw];
+    }
 #pragma unroll
     for (int q = 0; q < BFS_W; ++q) {
-      const int w = t + BFS_T * q;
-      if (w < nw) {
+      const int w = tw + BFS_T * q;
+      if (g == 0 && w < nw) {
         uint32_t fresh = acc[q] & ~visited[w];
         if (fresh) {
           visited[w] |= fresh;
@@ -199,11 +214,12 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
 __global__ __launch_bounds__(BFS_T) void bfs_sources_kernel(const uint32_t *adj, int n, int nw,
                                                             int *lists, int *res) {
   __shared__ uint32_t visited[BFS_T * BFS_W];
+  __shared__ uint32_t red[BFS_T];
   __shared__ int s_cnt[3];
   const int k = bid_x();
   const int src = (int)((int64_t)k * n / NSRC);
   int *cur = lists + (int64_t)k * 2 * n;
-  bfs(adj, n, nw, src, cur, cur + n, nullptr, res + 3 * k, visited, s_cnt);
+  bfs(adj, n, nw, src, cur, cur + n, nullptr, res + 3 * k, visited, red, s_cnt);
 }
 
 // Vertex levels for the order: the search from the lowest-numbered vertex of
@@ -214,15 +230,16 @@ __global__ __launch_bounds__(BFS_T) void bfs_levels_kernel(const uint32_t *adj, 
                                                            int *lists, int *level, int *res,
                                                            const int *far) {
   __shared__ uint32_t visited[BFS_T * BFS_W];
+  __shared__ uint32_t red[BFS_T];
   __shared__ int s_cnt[3];
   int src;
   if (far) {
     src = *far;
   } else {
-    bfs(adj, n, nw, 0, lists, lists + n, nullptr, res, visited, s_cnt);
+    bfs(adj, n, nw, 0, lists, lists + n, nullptr, res, visited, red, s_cnt);
     src = res[2];
   }
-  bfs(adj, n, nw, src, lists, lists + n, level, res + 3, visited, s_cnt);
+  bfs(adj, n, nw, src, lists, lists + n, level, res + 3, visited, red, s_cnt);
 }
 
 // The 24 features (one workgroup; fixed-order reductions in double).
@@ -476,7 +493,8 @@ hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int 
   if (e == hipSuccess) e = launch_fill_i32(sym, 0, 2, s);
   if (e != hipSuccess) return e;
   row_stats_kernel<<<dim3(n), dim3(256), 0, s>>>(X, ld, n, rstats);
-  const int chunk = 512;
+  // about 2048 workgroups whatever n: each thread walks `chunk` rows of one column
+  const int chunk = std::max(8, (int)(((int64_t)n * n) / (256 * 2048)));
   in_degree_kernel<<<dim3((n + 255) / 256, (n + chunk - 1) / chunk), dim3(256), 0, s>>>(
       X, ld, n, chunk, indeg);
   const int step = std::max(1, n / 256);

@@ -108,3 +108,32 @@ def test_learned_loop_is_bit_exact(gpu, oracle_learned, which):
         print(op, X.shape[0], itr, itr2, sched, rep["host_syncs"], flush=True)
         assert st == simd2_amd.SUCCESS and itr2 == itr
         assert torch.equal(Z.view(torch.int32), Y.view(torch.int32))
+
+
+def test_trained_network_replaces_flag_reads(gpu):
+    """What the trained network buys, measured without a clock: on held-out
+    graphs (the evaluation's generator and seed stream, n = 768) its predicted
+    squaring count lets the loop read the convergence flags once or twice where
+    the reference's loop structure reads them after every squaring, and the
+    prediction is never below the true count on most graphs; the results stay
+    bit-identical."""
+    torch = gpu
+    if not os.path.exists(LG.MODEL_PATH):
+        pytest.skip("no trained model file")
+    from simd2_amd.eval_schedule import held_out_specs
+    from simd2_amd.train_schedule import make_graph
+    model, _ = LG.load_model()
+    learned_reads = plain_reads = covered = 0
+    specs = held_out_specs(12)
+    for fam, op, epi, check, param, seed in specs:
+        X = make_graph(torch, fam, op, 768, param, seed)
+        Y, itr = plain(torch, op, X, epi)
+        Z, itr2, st, (tile, pred), rep = LG.converge_learned(op, X.clone(), model, epilogue=epi)
+        print(fam, op, itr, pred, rep["host_syncs"], flush=True)
+        assert st == simd2_amd.SUCCESS and itr2 == itr
+        assert torch.equal(Z.view(torch.int32), Y.view(torch.int32))
+        learned_reads += rep["host_syncs"]
+        plain_reads += itr
+        covered += int(pred >= itr)
+    assert covered >= 0.8 * len(specs), covered
+    assert learned_reads <= 0.4 * plain_reads, (learned_reads, plain_reads)
