@@ -149,7 +149,9 @@ struct SchedArgs {
   //   row-sparse (*sq_mode 1): the input's entries other than the
   //     (+)-identity, when there are at most rows_limit;
   //   changed-entry (*sq_mode 2): the input's entries that differ from
-  //     `prev` (the previous squaring's input), when at most changed_limit.
+  //     `prev` (the previous squaring's input), when at most changed_limit;
+  //     *sq_mode 3: the same list for a symmetric matrix (*asym == 0): row
+  //     passes only, then the result symmetrised in place.
   //   *sq_mode 0: the tiled kernel squares.  sq_mode null: tiled only.
   // A form whose *_auto flag is set is taken only when the cost model says
   // it beats the other forms: cost_slab_tile per (tile, slab) pair the tiled
@@ -157,6 +159,7 @@ struct SchedArgs {
   // per listed entry, cost_changed_entry per changed entry, plus cost_list
   // for either sparse form.
   const float *prev;  // null: no changed-entry form for this squaring
+  const int *asym;    // 0: the input of the loop is symmetric (null: unknown)
   int rows_limit, changed_limit;
   int rows_auto, changed_auto;
   float cost_slab_tile, cost_row_entry, cost_changed_entry, cost_list;
@@ -168,6 +171,9 @@ struct SchedArgs {
   float *lval;
 };
 hipError_t launch_sched_square(int op, int cfg, const SchedArgs &a, hipStream_t s);
+// *asym = 1 unless the n x n matrix D equals its transpose bit for bit
+// (*asym must be 0 beforehand).
+hipError_t launch_sched_symmetry(const float *D, int64_t ld, int n, int *asym, hipStream_t s);
 // After a sparse squaring (nothing after a tiled one): occupancy maps of
 // its output.  With a == null: of `D` for the first squaring.
 hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, int n,

@@ -264,20 +264,27 @@ __global__ __launch_bounds__(SCAN_T) void list_scan_kernel(SchedArgs a) {
     // both limits are within the list's capacity
     const bool rows_ok = a.rows_limit > 0 && tn <= a.rows_limit;
     const bool ch_ok = a.prev && a.changed_limit > 0 && tc <= a.changed_limit;
+    const bool sym = a.asym && *a.asym == 0;
     int mode = 0;
     if (rows_ok && !a.rows_auto) {
       mode = 1;
     } else if (ch_ok && !a.changed_auto) {
-      mode = 2;
+      mode = sym ? 3 : 2;
     } else {
       const double c_rows = (double)tn * a.cost_row_entry + a.cost_list;
       const double c_changed = (double)tc * a.cost_changed_entry + a.cost_list;
+      // symmetric: row passes over the changed entries, then a symmetrising pass
+      const double c_sym = (double)tc * a.cost_row_entry + 2.0 * a.cost_list;
       double best = (double)*a.work * a.cost_slab_tile;
       if (rows_ok && c_rows < best) {
         mode = 1;
         best = c_rows;
       }
-      if (ch_ok && c_changed < best) mode = 2;
+      if (ch_ok && c_changed < best) {
+        mode = 2;
+        best = c_changed;
+      }
+      if (ch_ok && sym && c_sym < best) mode = 3;
     }
     *a.sq_mode = mode;
     s_mode = mode;
@@ -361,13 +368,18 @@ __device__ __forceinline__ vf4 relax4(vf4 acc, float x0, vf4 y0, float x1, vf4 y
   return acc;
 }
 
-// One output row per workgroup; CHANGED selects the changed-entry form.
-template <class Op, bool CHANGED>
+// One output row per workgroup.  FORM 1: row-sparse; 2: changed-entry;
+// 3: changed-entry of a symmetric matrix, row passes only: it writes
+// R = D (+) (row passes) and no flags, and symmetrize_kernel finishes
+// D' = R (+) R^T (the scatter terms of row i are the row-pass terms of
+// column i when D and its changes are symmetric).
+template <class Op, int FORM>
 __global__ __launch_bounds__(EW) void sparse_square_kernel(SchedArgs a, int bm, int bn) {
+  constexpr bool CHANGED = FORM == 2;
   __shared__ float lds_acc[CHANGED ? SP_CH : 1];
   __shared__ float lds_x[CHANGED ? SP_CH : 1];
   if (a.conv_prev && *a.conv_prev == 0) return;
-  if (*a.sq_mode != (CHANGED ? 2 : 1)) return;
+  if (*a.sq_mode != FORM) return;
   const int i = bid_x(), tid = tid_x(), lane = tid % 64;
   const uint32_t ident = __float_as_uint(Op::identity);
   const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
@@ -451,7 +463,7 @@ __global__ __launch_bounds__(EW) void sparse_square_kernel(SchedArgs a, int bm, 
           float t = v[q];
           if constexpr (CHANGED) t = Op::add(t, lds_acc[c - jc + q]);
           if (a.epi) t = Op::add(t, p[q]);
-          if (!CHANGED && a.mode == CHECK_APLP && t > -1.0f && t < 1.0f) t = -FLT_MAX;
+          if (FORM == 1 && a.mode == CHECK_APLP && t > -1.0f && t < 1.0f) t = -FLT_MAX;
           v[q] = t;
           if (c + q < a.n) {
             col |= __float_as_uint(t) != __float_as_uint(p[q]);
@@ -459,7 +471,7 @@ __global__ __launch_bounds__(EW) void sparse_square_kernel(SchedArgs a, int bm, 
           }
         }
         chg_bits |= col;
-        if (col) a.colchg[c / bn] = 1;  // c .. c+3 lie in one column block
+        if (FORM != 3 && col) a.colchg[c / bn] = 1;  // c .. c+3 lie in one column block
         float *o = a.nxt + r0 + c;
         if (c + 4 <= a.n) {
           *reinterpret_cast<vf4 *>(o) = v;
@@ -472,12 +484,103 @@ __global__ __launch_bounds__(EW) void sparse_square_kernel(SchedArgs a, int bm, 
     }
     if constexpr (CHANGED) __syncthreads();  // lds_acc is reset by the next pass
   }
-  if (__any(chg_bits) && lane == 0) a.rowchg[i / bm] = 1;
-  if (__any(chg_test) && lane == 0) *a.conv = 1;
-  if (tid == 0 && a.stats) {
-    atomicAdd(a.stats + (CHANGED ? 4 : 2), 1ull);
-    atomicAdd(a.stats + (CHANGED ? 5 : 3), (unsigned long long)(e1 - e0));
+  if (FORM != 3) {  // the symmetrising pass sets the flags of form 3
+    if (__any(chg_bits) && lane == 0) a.rowchg[i / bm] = 1;
+    if (__any(chg_test) && lane == 0) *a.conv = 1;
   }
+  if (tid == 0 && a.stats) {
+    atomicAdd(a.stats + (FORM == 1 ? 2 : 4), 1ull);
+    atomicAdd(a.stats + (FORM == 1 ? 3 : 5), (unsigned long long)(e1 - e0));
+  }
+}
+
+// Pairs of SY x SY tiles (I, J), I <= J, one workgroup each, from the linear
+// workgroup index (tile row I holds T - I pairs).
+constexpr int SY = 64;
+__device__ __forceinline__ void tile_pair(int b, int T, int &I, int &J) {
+  I = 0;
+  while (b >= T - I) {
+    b -= T - I;
+    ++I;
+  }
+  J = I + b;
+}
+
+// Loads tile (I, J) of M into ta and tile (J, I) into tb (identity outside n).
+template <class Op>
+__device__ __forceinline__ void load_pair(const float *M, int64_t ld, int n, int r0, int c0,
+                                          float (*ta)[SY + 1], float (*tb)[SY + 1]) {
+  for (int e = tid_x(); e < SY * SY; e += EW) {
+    const int rl = e / SY, cl = e % SY;
+    ta[rl][cl] = r0 + rl < n && c0 + cl < n ? M[(int64_t)(r0 + rl) * ld + c0 + cl] : Op::identity;
+    tb[rl][cl] = c0 + rl < n && r0 + cl < n ? M[(int64_t)(c0 + rl) * ld + r0 + cl] : Op::identity;
+  }
+}
+
+// Form 3, second pass: D'[r][c] = R[r][c] (+) R[c][r] in place, with the
+// exact change flags and the convergence flag against the input.  A tile of
+// 64 lies inside one row block and one column block (bm, bn are 64 or 128).
+template <class Op>
+__global__ __launch_bounds__(EW) void symmetrize_kernel(SchedArgs a, int bm, int bn) {
+  __shared__ float ta[SY][SY + 1], tb[SY][SY + 1];
+  if (a.conv_prev && *a.conv_prev == 0) return;
+  if (*a.sq_mode != 3) return;
+  const int T = (a.n + SY - 1) / SY;
+  int I, J;
+  tile_pair(bid_x(), T, I, J);
+  const int r0 = I * SY, c0 = J * SY;
+  load_pair<Op>(a.nxt, a.ld, a.n, r0, c0, ta, tb);
+  __syncthreads();
+  bool chg1 = false, chg2 = false, test = false;
+  for (int e = tid_x(); e < SY * SY; e += EW) {
+    const int rl = e / SY, cl = e % SY;
+    if (r0 + rl < a.n && c0 + cl < a.n) {  // element (r0 + rl, c0 + cl) of tile (I, J)
+      const int64_t at = (int64_t)(r0 + rl) * a.ld + c0 + cl;
+      const float v = Op::add(ta[rl][cl], tb[cl][rl]), p = a.cur[at];
+      a.nxt[at] = v;
+      chg1 |= __float_as_uint(v) != __float_as_uint(p);
+      test |= differs(v, p, a.mode);
+    }
+    if (I != J && c0 + rl < a.n && r0 + cl < a.n) {  // element (c0 + rl, r0 + cl) of (J, I)
+      const int64_t at = (int64_t)(c0 + rl) * a.ld + r0 + cl;
+      const float v = Op::add(tb[rl][cl], ta[cl][rl]), p = a.cur[at];
+      a.nxt[at] = v;
+      chg2 |= __float_as_uint(v) != __float_as_uint(p);
+      test |= differs(v, p, a.mode);
+    }
+  }
+  chg1 = __syncthreads_or(chg1);
+  chg2 = __syncthreads_or(chg2);
+  test = __syncthreads_or(test);
+  if (tid_x() == 0) {
+    if (chg1) {
+      a.rowchg[r0 / bm] = 1;
+      a.colchg[c0 / bn] = 1;
+    }
+    if (chg2) {
+      a.rowchg[c0 / bm] = 1;
+      a.colchg[r0 / bn] = 1;
+    }
+    if (test) *a.conv = 1;
+  }
+}
+
+// *asym = 1 when some D[r][c] differs from D[c][r] in its bits.
+__global__ __launch_bounds__(EW) void symmetry_check_kernel(const float *D, int64_t ld, int n,
+                                                            int *asym) {
+  __shared__ float ta[SY][SY + 1], tb[SY][SY + 1];
+  const int T = (n + SY - 1) / SY;
+  int I, J;
+  tile_pair(bid_x(), T, I, J);
+  const int r0 = I * SY, c0 = J * SY;
+  load_pair<MinPlus>(D, ld, n, r0, c0, ta, tb);
+  __syncthreads();
+  bool diff = false;
+  for (int e = tid_x(); e < SY * SY; e += EW) {
+    const int rl = e / SY, cl = e % SY;
+    diff |= __float_as_uint(ta[rl][cl]) != __float_as_uint(tb[cl][rl]);
+  }
+  if (__syncthreads_or(diff) && tid_x() == 0) *asym = 1;
 }
 
 // Occupancy maps of a matrix, one workgroup per bm x bn tile (the maps the
@@ -555,8 +658,13 @@ hipError_t sched_op(int cfg, const SchedArgs &a, hipStream_t s) {
     list_scan_kernel<<<dim3(1), dim3(SCAN_T), 0, s>>>(a);
     list_fill_kernel<Op><<<dim3(a.n), dim3(EW), 0, s>>>(a);
     if (a.rows_limit > 0)
-      sparse_square_kernel<Op, false><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
-    if (a.prev) sparse_square_kernel<Op, true><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
+      sparse_square_kernel<Op, 1><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
+    if (a.prev) sparse_square_kernel<Op, 2><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
+    if (a.prev && a.asym) {
+      const int T = (a.n + SY - 1) / SY;
+      sparse_square_kernel<Op, 3><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm, c.bn);
+      symmetrize_kernel<Op><<<dim3(T * (T + 1) / 2), dim3(EW), 0, s>>>(a, c.bm, c.bn);
+    }
   }
   if (cfg == CFG_S64)
     sched_kernel<Op, CfgS><<<grid, block, 0, s>>>(a);
@@ -589,6 +697,13 @@ hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, i
   occupancy_kernel<<<dim3(tiles_r * tiles_c), dim3(EW), 0, s>>>(
       D, ld, n, c.bm, c.bn, tiles_c, slabs, bits, xocc, yocc, a ? a->conv_prev : nullptr,
       a ? a->sq_mode : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_sched_symmetry(const float *D, int64_t ld, int n, int *asym, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int T = (n + SY - 1) / SY;
+  symmetry_check_kernel<<<dim3(T * (T + 1) / 2), dim3(EW), 0, s>>>(D, ld, n, asym);
   return hipGetLastError();
 }
 

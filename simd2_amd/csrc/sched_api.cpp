@@ -52,7 +52,7 @@ constexpr double PS_ROW_ENTRY_COL = 0.507, PS_CHANGED_ENTRY_N = 1.8, PS_LIST_ELE
 struct Layout {
   int P, kpad, tiles_r, tiles_c, slabs;
   size_t buf[2], xocc[2], yocc[2], rowchg, colchg, conv, work, mode, rowptr, chgcnt, lkj, lval,
-      stats, total;
+      asym, stats, total;
 };
 
 bool layout(int n, int cfg, int max_itr, Layout *L) {
@@ -88,6 +88,7 @@ bool layout(int n, int cfg, int max_itr, Layout *L) {
   const size_t cap = (size_t)list_per_row(n) * n;
   L->lkj = take(cap * sizeof(uint32_t));
   L->lval = take(cap * sizeof(float));
+  L->asym = take(sizeof(int));
   L->stats = take(6 * sizeof(unsigned long long));
   L->total = off;
   return true;
@@ -150,6 +151,7 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   int *mode = reinterpret_cast<int *>(ws + L.mode);
   int *rowptr = reinterpret_cast<int *>(ws + L.rowptr);
   int *chgcnt = reinterpret_cast<int *>(ws + L.chgcnt);
+  int *asym = reinterpret_cast<int *>(ws + L.asym);
   uint32_t *lkj = reinterpret_cast<uint32_t *>(ws + L.lkj);
   float *lval = reinterpret_cast<float *>(ws + L.lval);
   unsigned long long *stats = reinterpret_cast<unsigned long long *>(ws + L.stats);
@@ -167,6 +169,10 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   if (e == hipSuccess)
     e = launch_fill_i32(reinterpret_cast<int *>(work), 0, 2 * (int64_t)(max_itr + 1), stream);
   if (e == hipSuccess) e = launch_fill_i32(mode, 0, max_itr + 1, stream);
+  // a symmetric input stays symmetric (every (x) here commutes bit for bit),
+  // which the changed-entry form can use
+  if (e == hipSuccess && changed_on) e = launch_fill_i32(asym, 0, 1, stream);
+  if (e == hipSuccess && changed_on) e = launch_sched_symmetry(buf[0], L.P, n, asym, stream);
   if (e == hipSuccess) e = launch_fill_i32(reinterpret_cast<int *>(stats), 0, 12, stream);
   if (e == hipSuccess)
     e = launch_sched_occupancy(op, cfg, buf[0], L.P, n, L.tiles_r, L.tiles_c, L.slabs, xocc[0],
@@ -205,6 +211,7 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
       // the previous squaring's input is the buffer this one overwrites; it
       // is read (to list the changed entries) before anything is written
       a.prev = changed_on && s > 0 ? a.nxt : nullptr;
+      a.asym = changed_on ? asym : nullptr;
       a.rows_limit = rows_on ? per_row * n : 0;
       a.changed_limit = a.prev ? changed_per_row * n : 0;
       a.rows_auto = rows_auto ? 1 : 0;
