@@ -129,3 +129,23 @@ def test_cap_and_edge_sizes(gpu):
         Y, itr, _ = plain(torch, "min_plus", X, True, "threshold")
         Z, itr2, _, _ = converge_scheduled("min_plus", X.clone(), predicted_itr=5)
         assert itr2 == itr and torch.equal(Z, Y), n
+
+
+def test_sparse_forms_past_one_column_pass(gpu):
+    """n above 8192: the sparse kernels take a row in more than one pass of
+    8192 columns, the changed-entry scatter its list in more than one chunk,
+    and the symmetrising pass ends on a partial tile pair."""
+    torch = gpu
+    n = 8192 + 300
+    for kind, X in (("er", er_family(torch, "min_plus", n, 4.0 / n, 5)),
+                    ("grid", grid_family(torch, "min_plus", n, 6))):
+        Y, itr, st = plain(torch, "min_plus", X, True, "threshold")
+        assert st == simd2_amd.SUCCESS
+        for rs, cs, key in ((1024, 0, "rows_sparse"), (0, 1024, "changed_rows")):
+            print(f"{kind} n={n} row_sparse={rs} changed_sparse={cs}", flush=True)
+            Z, itr2, _, rep = converge_scheduled("min_plus", X.clone(), tile_config=0,
+                                                 predicted_itr=itr, row_sparse=rs,
+                                                 changed_sparse=cs)
+            assert itr2 == itr and torch.equal(Z.view(torch.int32), Y.view(torch.int32)), rep
+            assert rep[key] >= n, rep
+        del Y, Z
