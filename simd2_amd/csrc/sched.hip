@@ -177,42 +177,41 @@ constexpr int SP_B = 8;                // changed entries per thread per batch
 
 // (tile, slab) pairs the tiled kernel would compute in this squaring: dirty
 // tiles times the slabs occupied on both sides, plus two per dirty tile for
-// its epilogue.  One thread per tile.
+// its epilogue.  One wave per tile, its lanes over the slabs.
 __global__ __launch_bounds__(EW) void tiled_work_kernel(SchedArgs a) {
   if (a.conv_prev && *a.conv_prev == 0) return;
-  const int t = bid_x() * EW + tid_x();
-  unsigned long long w = 0;
-  if (t < a.tiles_r * a.tiles_c) {
-    const int tr = t / a.tiles_c, tc = t % a.tiles_c;
-    if (!a.rowchg_prev || a.rowchg_prev[tr] || a.colchg_prev[tc]) {
-      const uint8_t *xo = a.xocc_in + (int64_t)tr * a.slabs;
-      const uint8_t *yo = a.yocc_in + (int64_t)tc * a.slabs;
-      int cnt = 2;
-      for (int s = 0; s < a.slabs; ++s) cnt += xo[s] & yo[s];
-      w = (unsigned long long)cnt;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
-  if (tid_x() % 64 == 0 && w) atomicAdd(a.work, w);
+  const int t = bid_x() * (EW / 64) + tid_x() / 64, lane = tid_x() % 64;
+  if (t >= a.tiles_r * a.tiles_c) return;  // whole waves leave
+  const int tr = t / a.tiles_c, tc = t % a.tiles_c;
+  if (a.rowchg_prev && !a.rowchg_prev[tr] && !a.colchg_prev[tc]) return;  // clean tile
+  const uint8_t *xo = a.xocc_in + (int64_t)tr * a.slabs;
+  const uint8_t *yo = a.yocc_in + (int64_t)tc * a.slabs;
+  int cnt = 0;
+  for (int s = lane; s < a.slabs; s += 64) cnt += xo[s] & yo[s];
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (lane == 0) atomicAdd(a.work, (unsigned long long)(cnt + 2));
 }
 
-// Per row: entries other than the identity (rowptr[i]) and, with `prev`,
-// entries that differ from prev (chgcnt[i]).
+// Per row: entries other than the identity (rowptr[i], when the row-sparse
+// form is on) and, with `prev`, entries that differ from prev (chgcnt[i]; a
+// row of a tile row the previous squaring left unchanged has none).
 template <class Op>
-__global__ __launch_bounds__(EW) void list_count_kernel(SchedArgs a) {
+__global__ __launch_bounds__(EW) void list_count_kernel(SchedArgs a, int bm) {
   __shared__ int s_wave[2][EW / 64];
   if (a.conv_prev && *a.conv_prev == 0) return;
   const int i = bid_x(), tid = tid_x(), lane = tid % 64, wave = tid / 64;
   const uint32_t ident = __float_as_uint(Op::identity);
   const int64_t r0 = (int64_t)i * a.ld;
+  const bool need_nz = a.rows_limit > 0;
+  const bool need_ch = a.prev && !(a.rowchg_prev && !a.rowchg_prev[i / bm]);
   int nz = 0, ch = 0;
   // columns n .. ld-1 hold the identity in every buffer, so whole groups of
   // four can be tested (ld is a multiple of 128)
-  for (int c = 4 * tid; c < a.n; c += 4 * EW) {
+  for (int c = 4 * tid; (need_nz || need_ch) && c < a.n; c += 4 * EW) {
     const vf4 x = *reinterpret_cast<const vf4 *>(a.cur + r0 + c);
     nz += (__float_as_uint(x.x) != ident) + (__float_as_uint(x.y) != ident) +
           (__float_as_uint(x.z) != ident) + (__float_as_uint(x.w) != ident);
-    if (a.prev) {
+    if (need_ch) {
       const vf4 y = *reinterpret_cast<const vf4 *>(a.prev + r0 + c);
       ch += (__float_as_uint(x.x) != __float_as_uint(y.x)) +
             (__float_as_uint(x.y) != __float_as_uint(y.y)) +
@@ -653,8 +652,8 @@ hipError_t sched_op(int cfg, const SchedArgs &a, hipStream_t s) {
   if (a.sq_mode) {  // the list, then whichever of the three forms it chose works
     const ValuCfgInfo c = valu_cfg_info(cfg);
     const int tiles = a.tiles_r * a.tiles_c;
-    tiled_work_kernel<<<dim3((tiles + EW - 1) / EW), dim3(EW), 0, s>>>(a);
-    list_count_kernel<Op><<<dim3(a.n), dim3(EW), 0, s>>>(a);
+    tiled_work_kernel<<<dim3((tiles + EW / 64 - 1) / (EW / 64)), dim3(EW), 0, s>>>(a);
+    list_count_kernel<Op><<<dim3(a.n), dim3(EW), 0, s>>>(a, c.bm);
     list_scan_kernel<<<dim3(1), dim3(SCAN_T), 0, s>>>(a);
     list_fill_kernel<Op><<<dim3(a.n), dim3(EW), 0, s>>>(a);
     if (a.rows_limit > 0)
