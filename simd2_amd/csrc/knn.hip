@@ -19,7 +19,8 @@
 //      queries (each wave 16, their A fragments held in registers for the whole
 //      scan) and one slice of the references, streamed through LDS in 64-row
 //      tiles.  The 64x64 distance tile goes to LDS, where each thread scans 16
-//      references for one query and keeps the 16 smallest in registers.  The
+//      references for one query and keeps the 16 smallest in registers
+//      (through a small pending buffer merged in lockstep).  The
 //      Q x R matrix is never written; each query leaves 64 candidates per
 //      reference slice;
 //    * rerank_kernel: one wave per query picks the 32 best candidates by the
@@ -169,6 +170,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, 
 constexpr int CQ = 64;    // queries per workgroup
 constexpr int CR = 64;    // references per tile
 constexpr int CLIST = 16; // candidates kept per thread
+constexpr int CPEND = 4;  // pending candidates per thread before a merge
 
 // q.r ~ qh.rh + qh.rl + ql.rh: three BF16 MFMAs per k-step carry ~16
 // significant bits, which the norm expansion needs (centres ~N(0, 10^2) make
@@ -235,6 +237,39 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
   float lmax = __builtin_inff();
   int lpos = 0;
   const int sq = tid / 4, sh = tid % 4;
+  // Candidates pass `dd < lmax` into a pending buffer of CPEND (a few
+  // instructions per reference); the buffers are merged into the lists, in
+  // arrival order and against the then-current lmax, when some lane's buffer
+  // is full -- all lanes at once -- so the list update does not run for the
+  // whole wave every time any single lane has a candidate.  The lists come
+  // out exactly as with one-at-a-time insertion: lmax only falls.
+  float pd[CPEND];
+  int pi[CPEND];
+  int pc = 0;
+  auto insert = [&](float dd, int idx) {
+    if (dd < lmax) {
+#pragma unroll
+      for (int c = 0; c < CLIST; ++c)
+        if (c == lpos) {
+          ld[c] = dd;
+          li[c] = idx;
+        }
+      lmax = ld[0];
+      lpos = 0;
+#pragma unroll
+      for (int c = 1; c < CLIST; ++c)
+        if (ld[c] > lmax) {
+          lmax = ld[c];
+          lpos = c;
+        }
+    }
+  };
+  auto flush = [&]() {
+#pragma unroll
+    for (int b = 0; b < CPEND; ++b)
+      if (b < pc) insert(pd[b], pi[b]);
+    pc = 0;
+  };
 
   // Register prefetch of the next reference tile (global -> registers while
   // the current tile is multiplied and scanned, registers -> LDS after).
@@ -304,22 +339,17 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
       if (dd < lmax) {
         const int idx = r0 + sh * (CR / 4) + j;
 #pragma unroll
-        for (int c = 0; c < CLIST; ++c)
-          if (c == lpos) {
-            ld[c] = dd;
-            li[c] = idx;
+        for (int b = 0; b < CPEND; ++b)
+          if (b == pc) {
+            pd[b] = dd;
+            pi[b] = idx;
           }
-        lmax = ld[0];
-        lpos = 0;
-#pragma unroll
-        for (int c = 1; c < CLIST; ++c)
-          if (ld[c] > lmax) {
-            lmax = ld[c];
-            lpos = c;
-          }
+        ++pc;
       }
+      if (__any(pc == CPEND)) flush();
     }
   }
+  flush();
   const int q = q0 + sq;
   if (q < Q) {
     const int64_t base = (int64_t)q * ncand + (int64_t)bid_y() * (4 * CLIST) + sh * CLIST;
@@ -467,8 +497,9 @@ hipError_t launch_knn_select(const float *dist, int Q, int R, int k, float *out_
 size_t knn_bf16_workspace(int Q, int R, int d, int *slices_out) {
   const int dpad = dpad_of(d);
   const int qblocks = (Q + CQ - 1) / CQ;
-  // Enough reference slices to give ~768 workgroups (3 per CU) in one round.
-  int slices = (768 + qblocks - 1) / qblocks;
+  // Enough reference slices to fill the GPU in whole rounds: the kernel
+  // holds 2 workgroups per CU (its registers), 512 at a time.
+  int slices = (1024 + qblocks - 1) / qblocks;
   slices = slices < 1 ? 1 : (slices > 8 ? 8 : slices);
   const int max_slices = (R + CR - 1) / CR;
   if (slices > max_slices) slices = max_slices < 1 ? 1 : max_slices;
