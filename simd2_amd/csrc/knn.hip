@@ -13,16 +13,21 @@
 // 2. The precision form (no counterpart in the reference):
 //    * prep_kernel: FP32 points -> BF16 high and low parts (zero-padded to a
 //      multiple of 32 dims) and FP32 squared norms;
-//    * cand_kernel: ||q||^2 + ||r||^2 - 2 q.r with the dot product on BF16 MFMA
-//      (v_mfma_f32_16x16x32_bf16, FP32 accumulate; hi.hi + hi.lo + lo.hi, three
-//      MFMAs per k-step, for ~16 significant bits).  A workgroup owns 64
-//      queries (each wave 16, their A fragments held in registers for the whole
+//    * knn_pass_kernel: ||q||^2 + ||r||^2 - 2 q.r with the dot product on BF16
+//      MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate; hi.hi + hi.lo + lo.hi,
+//      three MFMAs per k-step, for ~16 significant bits).  A workgroup owns 128
+//      queries (each wave 32, their A fragments held in registers for the whole
 //      scan) and one slice of the references, streamed through LDS in 64-row
-//      tiles.  The 64x64 distance tile goes to LDS, where each thread scans 16
-//      references for one query and keeps the 16 smallest in registers
-//      (through a small pending buffer merged in lockstep).  The
-//      Q x R matrix is never written; each query leaves 64 candidates per
-//      reference slice;
+//      tiles; the distances stay in the MFMA accumulators (lane l: 4 queries x
+//      4 references of a tile).  Two passes over the same products:
+//        pass 1 writes, per (query, tile), the smallest distance of the tile;
+//        knn_tau_kernel takes each query's CAND_K-th smallest tile minimum,
+//          tau: CAND_K distinct references lie at or under it, so it bounds
+//          the query's CAND_K-th best distance from above;
+//        pass 2 appends every reference at or under tau to the query's
+//          candidate buffer (rare: about CAND_K to 2 CAND_K per query).
+//      Neither pass keeps lists in the inner loop, and the Q x R matrix is
+//      never written;
 //    * rerank_kernel: one wave per query picks the 32 best candidates by the
 //      approximate distance, recomputes them exactly in FP32 ((q - r)^2 summed
 //      by the wave), ranks by (distance, index), and for candidates whose FP32
@@ -167,112 +172,84 @@ __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, 
   if (lane == 0) norm[p] = s;
 }
 
-constexpr int CQ = 64;    // queries per workgroup
-constexpr int CR = 64;    // references per tile
-constexpr int CLIST = 16; // candidates kept per thread
-constexpr int CPEND = 4;  // pending candidates per thread before a merge
+constexpr int QG = 2;       // groups of 16 queries per wave
+constexpr int CQ = 64 * QG; // queries per workgroup (4 waves)
+constexpr int CR = 64;      // references per tile
+constexpr int CAND_K = 32;  // tau bounds the CAND_K-th best distance
+// Candidate buffer per query: with no ties at tau at most CAND_K tiles hold
+// references at or under it, 64 each; a pathological tie of more tile
+// minima at exactly tau keeps the first CAND_CAP arrivals.
+constexpr int CAND_CAP = CAND_K * CR;
 
 // q.r ~ qh.rh + qh.rl + ql.rh: three BF16 MFMAs per k-step carry ~16
 // significant bits, which the norm expansion needs (centres ~N(0, 10^2) make
 // ||q||^2 + ||r||^2 ~ 100x the neighbour distances); a single BF16 product
 // leaves errors of ~10 on distances ~250 and loses true neighbours.
-template <int DPAD>
-__global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb,
-                                                   const __bf16 *__restrict__ Rb,
-                                                   const float *__restrict__ qn,
-                                                   const float *__restrict__ rn, int Q, int R,
-                                                   int slice, float *cand_d, int *cand_i,
-                                                   int ncand) {
+// PASS 1: tmin[q][tile] = min over the tile; PASS 2: append r with d <= tau[q].
+template <int DPAD, int PASS>
+__global__ __launch_bounds__(256) void knn_pass_kernel(
+    const __bf16 *__restrict__ Qb, const __bf16 *__restrict__ Rb, const float *__restrict__ qn,
+    const float *__restrict__ rn, int Q, int R, int slices, int slice, float *tmin, int ntiles,
+    const float *__restrict__ tau, float *cand_d, int *cand_i, int *cand_n) {
   constexpr int KS = DPAD / 32;             // MFMA k-steps
   constexpr int RLD = 2 * DPAD + 8;         // LDS row: hi | lo, +16 bytes of padding
   constexpr int CHUNKS = CR * 2 * DPAD / 8; // 16-byte chunks per reference tile
   constexpr int PER = CHUNKS / 256;         // chunks per thread
   __shared__ __attribute__((aligned(16))) __bf16 Rt[CR][RLD];
-  __shared__ float Dt[CQ][CR + 1];
   __shared__ float Rn[CR];
+  constexpr int CB = PASS == 2 ? 16 : 1;  // LDS candidate slots per query
+  __shared__ float Cd[CQ][CB];
+  __shared__ int Ci[CQ][CB];
+  __shared__ int Cn[CQ];
 
   const int tid = tid_x();
   const int lane = tid % 64;
   const int wave = tid / 64;
-  const int q0 = bid_x() * CQ;
-  const int rb0 = bid_y() * slice;
+  if (PASS == 2)
+    for (int i = tid; i < CQ; i += 256) Cn[i] = 0;  // published by the loop's first barrier
+  // Workgroup b takes query block b / slices and reference slice b % slices:
+  // with 8 slices, workgroups run on XCD b % 8, so each XCD streams one slice
+  // and its L2 keeps it.
+  const int q0 = (bid_x() / slices) * CQ;
+  const int rb0 = (bid_x() % slices) * slice;
   const int rb1 = min(R, rb0 + slice);
 
-  // A fragments of this wave's 16 queries: lane l holds query l%16, dims
-  // 8*(l/16) .. +7 of every 32-wide k-step, high and low parts.
-  bf16x8 ah[KS], al[KS];
-  {
-    const int q = q0 + wave * 16 + lane % 16;
+  // A fragments of this wave's QG groups of 16 queries (every reference
+  // fragment read from LDS feeds QG x 3 MFMAs): lane l holds query l%16 of
+  // each group, dims 8*(l/16) .. +7 of every 32-wide k-step, high and low.
+  bf16x8 ah[QG][KS], al[QG][KS];
+#pragma unroll
+  for (int g = 0; g < QG; ++g) {
+    const int q = q0 + wave * 16 * QG + g * 16 + lane % 16;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       if (q < Q) {
         const __bf16 *base = Qb + (int64_t)q * 2 * DPAD + ks * 32 + 8 * (lane / 16);
-        ah[ks] = *reinterpret_cast<const bf16x8 *>(base);
-        al[ks] = *reinterpret_cast<const bf16x8 *>(base + DPAD);
+        ah[g][ks] = *reinterpret_cast<const bf16x8 *>(base);
+        al[g][ks] = *reinterpret_cast<const bf16x8 *>(base + DPAD);
       } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          ah[ks][e] = (__bf16)0.0f;
-          al[ks][e] = (__bf16)0.0f;
+          ah[g][ks][e] = (__bf16)0.0f;
+          al[g][ks][e] = (__bf16)0.0f;
         }
       }
     }
   }
-  // Norms of the 4 queries whose accumulator rows this lane holds.
-  float qnv[4];
+  // The 4 queries per group whose accumulator rows this lane holds: norms and tau.
+  float qnv[QG][4], tv[QG][4];
+  int qv[QG][4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int q = q0 + wave * 16 + 4 * (lane / 16) + v;
-    qnv[v] = q < Q ? qn[q] : 0.f;
-  }
-
-  // Candidate list of thread (query tid/4, tile quarter tid%4).
-  float ld[CLIST];
-  int li[CLIST];
+  for (int g = 0; g < QG; ++g)
 #pragma unroll
-  for (int c = 0; c < CLIST; ++c) {
-    ld[c] = __builtin_inff();
-    li[c] = -1;
-  }
-  float lmax = __builtin_inff();
-  int lpos = 0;
-  const int sq = tid / 4, sh = tid % 4;
-  // Candidates pass `dd < lmax` into a pending buffer of CPEND (a few
-  // instructions per reference); the buffers are merged into the lists, in
-  // arrival order and against the then-current lmax, when some lane's buffer
-  // is full -- all lanes at once -- so the list update does not run for the
-  // whole wave every time any single lane has a candidate.  The lists come
-  // out exactly as with one-at-a-time insertion: lmax only falls.
-  float pd[CPEND];
-  int pi[CPEND];
-  int pc = 0;
-  auto insert = [&](float dd, int idx) {
-    if (dd < lmax) {
-#pragma unroll
-      for (int c = 0; c < CLIST; ++c)
-        if (c == lpos) {
-          ld[c] = dd;
-          li[c] = idx;
-        }
-      lmax = ld[0];
-      lpos = 0;
-#pragma unroll
-      for (int c = 1; c < CLIST; ++c)
-        if (ld[c] > lmax) {
-          lmax = ld[c];
-          lpos = c;
-        }
+    for (int v = 0; v < 4; ++v) {
+      qv[g][v] = q0 + wave * 16 * QG + g * 16 + 4 * (lane / 16) + v;
+      qnv[g][v] = qv[g][v] < Q ? qn[qv[g][v]] : 0.f;
+      tv[g][v] = PASS == 2 && qv[g][v] < Q ? tau[qv[g][v]] : -__builtin_inff();
     }
-  };
-  auto flush = [&]() {
-#pragma unroll
-    for (int b = 0; b < CPEND; ++b)
-      if (b < pc) insert(pd[b], pi[b]);
-    pc = 0;
-  };
 
   // Register prefetch of the next reference tile (global -> registers while
-  // the current tile is multiplied and scanned, registers -> LDS after).
+  // the current tile is multiplied, registers -> LDS after).
   uint4 pre[PER];
   float pren = 0.f;
   auto fetch = [&](int r0) {
@@ -300,13 +277,15 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
 
   if (rb0 < rb1) fetch(rb0);
   for (int r0 = rb0; r0 < rb1; r0 += CR) {
-    __syncthreads();  // previous tile fully multiplied and scanned
+    __syncthreads();  // previous tile fully multiplied
     stage();
     __syncthreads();
     if (r0 + CR < rb1) fetch(r0 + CR);
-    f32x4 acc[4];
+    f32x4 acc[QG][4];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) acc[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int g = 0; g < QG; ++g)
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) acc[g][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
 #pragma unroll
@@ -314,50 +293,129 @@ __global__ __launch_bounds__(256) void cand_kernel(const __bf16 *__restrict__ Qb
         const __bf16 *row = &Rt[rb * 16 + lane % 16][ks * 32 + 8 * (lane / 16)];
         const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(row);
         const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(row + DPAD);
-        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ks], bh, acc[rb], 0, 0, 0);
-        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bl, acc[rb], 0, 0, 0);
-        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ks], bh, acc[rb], 0, 0, 0);
+#pragma unroll
+        for (int g = 0; g < QG; ++g) {
+          acc[g][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[g][ks], bh, acc[g][rb], 0, 0, 0);
+          acc[g][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[g][ks], bl, acc[g][rb], 0, 0, 0);
+          acc[g][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[g][ks], bh, acc[g][rb], 0, 0, 0);
+        }
       }
     }
-    // Approximate distances into the LDS tile; references past the slice end
-    // get +inf so they never become candidates.
+    // Distances of this lane's queries x references rb*16 + lane%16;
+    // references past the slice end take no part.
+    float rnv[4];
+    bool valid[4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
-      const int rr = rb * 16 + lane % 16;
-      const bool valid = r0 + rr < rb1;
-      const float rnv = Rn[rr];
+      rnv[rb] = Rn[rb * 16 + lane % 16];
+      valid[rb] = r0 + rb * 16 + lane % 16 < rb1;
+    }
+#pragma unroll
+    for (int g = 0; g < QG; ++g)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const float dd = valid ? qnv[v] + rnv - 2.0f * acc[rb][v] : __builtin_inff();
-        Dt[wave * 16 + 4 * (lane / 16) + v][rr] = dd;
-      }
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int j = 0; j < CR / 4; ++j) {
-      const float dd = Dt[sq][sh * (CR / 4) + j];
-      if (dd < lmax) {
-        const int idx = r0 + sh * (CR / 4) + j;
+        if constexpr (PASS == 1) {  // the tile minimum of each query
+          float m = __builtin_inff();
 #pragma unroll
-        for (int b = 0; b < CPEND; ++b)
-          if (b == pc) {
-            pd[b] = dd;
-            pi[b] = idx;
-          }
-        ++pc;
+          for (int rb = 0; rb < 4; ++rb)
+            if (valid[rb]) m = fminf(m, qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v]);
+          // the 16 lanes of this query (same lane / 16) hold the tile's 64 references
+#pragma unroll
+          for (int off = 1; off < 16; off <<= 1) m = fminf(m, __shfl_xor(m, off));
+          if (lane % 16 == 0 && qv[g][v] < Q) tmin[(int64_t)qv[g][v] * ntiles + r0 / CR] = m;
+        }
       }
-      if (__any(pc == CPEND)) flush();
+    if constexpr (PASS == 2) {
+      // References at or under tau (about CAND_K per query over all slices)
+      // take a slot of the query's LDS buffer; overflow goes straight to the
+      // global buffer.
+#pragma unroll
+      for (int g = 0; g < QG; ++g)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int rb = 0; rb < 4; ++rb) {
+            const float dd = qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v];
+            if (valid[rb] && dd <= tv[g][v]) {
+              const int q = qv[g][v], ql = q - q0;
+              const int at = atomicAdd(&Cn[ql], 1);
+              const int r = r0 + rb * 16 + lane % 16;
+              if (at < CB) {
+                Cd[ql][at] = dd;
+                Ci[ql][at] = r;
+              } else {
+                const int gat = atomicAdd(&cand_n[q], 1);
+                if (gat < CAND_CAP) {
+                  cand_d[(int64_t)q * CAND_CAP + gat] = dd;
+                  cand_i[(int64_t)q * CAND_CAP + gat] = r;
+                }
+              }
+            }
+          }
     }
   }
-  flush();
-  const int q = q0 + sq;
-  if (q < Q) {
-    const int64_t base = (int64_t)q * ncand + (int64_t)bid_y() * (4 * CLIST) + sh * CLIST;
-#pragma unroll
-    for (int c = 0; c < CLIST; ++c) {
-      cand_d[base + c] = ld[c];
-      cand_i[base + c] = li[c];
+  if constexpr (PASS == 2) {
+    // The LDS buffers to the global ones: one atomic per query.
+    __syncthreads();
+    for (int ql = tid; ql < CQ; ql += 256) {
+      const int q = q0 + ql, nb = min(Cn[ql], CB);
+      if (q < Q && nb > 0) {
+        const int b = atomicAdd(&cand_n[q], nb);
+        for (int e = 0; e < nb && b + e < CAND_CAP; ++e) {
+          cand_d[(int64_t)q * CAND_CAP + b + e] = Cd[ql][e];
+          cand_i[(int64_t)q * CAND_CAP + b + e] = Ci[ql][e];
+        }
+      }
     }
+  }
+}
+
+// Order-preserving 32-bit key of a (non-NaN) float.
+__device__ __forceinline__ uint32_t fkey(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_inv(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// tau[q] = the CAND_K-th smallest of the query's tile minima (+inf when it has
+// fewer tiles), by bisection over the keys; one wave per query.  Also clears
+// the query's candidate count.
+__global__ __launch_bounds__(256) void knn_tau_kernel(const float *__restrict__ tmin, int ntiles,
+                                                      int Q, float *tau, int *cand_n) {
+  const int lane = tid_x() % 64;
+  const int q = bid_x() * 4 + tid_x() / 64;
+  if (q >= Q) return;
+  const float *t = tmin + (int64_t)q * ntiles;
+  uint32_t lo = 0u, hi = 0xffffffffu;  // smallest key with >= CAND_K minima at or under it
+  constexpr int TP = 8;                 // keys held in registers per lane (ntiles <= 512)
+  uint32_t kv[TP];
+#pragma unroll
+  for (int i = 0; i < TP; ++i) {
+    const int j = lane + 64 * i;
+    kv[i] = j < ntiles ? fkey(t[j]) : 0xffffffffu;
+  }
+  if (ntiles < CAND_K) {
+    lo = fkey(__builtin_inff());
+  } else {
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      int c = 0;
+#pragma unroll
+      for (int i = 0; i < TP; ++i) c += (lane + 64 * i < ntiles && kv[i] <= mid) ? 1 : 0;
+      for (int j = lane + 64 * TP; j < ntiles; j += 64) c += fkey(t[j]) <= mid ? 1 : 0;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+      if (c >= CAND_K)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+  }
+  if (lane == 0) {
+    tau[q] = fkey_inv(lo);
+    cand_n[q] = 0;
   }
 }
 
@@ -375,7 +433,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float *__restrict__ Q
                                                      const float *__restrict__ Rf, int64_t ldr,
                                                      int Q, int d, int k,
                                                      const float *__restrict__ cand_d,
-                                                     const int *__restrict__ cand_i, int ncand,
+                                                     const int *__restrict__ cand_i,
+                                                     const int *__restrict__ cand_n, int ncand,
                                                      float *out_d, int *out_i, int do_sqrt,
                                                      int *stats) {
   const int lane = tid_x() % 64;
@@ -383,15 +442,36 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float *__restrict__ Q
   if (q >= Q) return;
   const float *cd = cand_d + (int64_t)q * ncand;
   const int *ci = cand_i + (int64_t)q * ncand;
+  const int nq = min(cand_n[q], ncand);  // candidates of this query
 
   // Step 1: the RR_M best candidates by (approximate distance, index).
   int sel = -1;  // lane m < RR_M receives the m-th choice
+  if (nq <= 64) {
+    // one candidate per lane, bitonic sort of the wave by the key
+    float bd = lane < nq ? cd[lane] : __builtin_inff();
+    int bi = lane < nq ? ci[lane] : 0x7fffffff;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const float od = __shfl_xor(bd, j);
+        const int oi = __shfl_xor(bi, j);
+        // the lower lane of the pair keeps the smaller key in an ascending run
+        const bool up = (lane & k) == 0, low = (lane & j) == 0;
+        const bool take = (low == up) ? key_less(od, oi, bd, bi) : key_less(bd, bi, od, oi);
+        if (take) {
+          bd = od;
+          bi = oi;
+        }
+      }
+    if (lane < RR_M && bi != 0x7fffffff) sel = bi;
+  }
   float last_d = -__builtin_inff();
   int last_i = -1;
-  for (int m = 0; m < RR_M; ++m) {
+  for (int m = 0; nq > 64 && m < RR_M; ++m) {
     float bd = __builtin_inff();
     int bi = 0x7fffffff;
-    for (int c = lane; c < ncand; c += 64) {
+    for (int c = lane; c < nq; c += 64) {
       const float dd = cd[c];
       const int ii = ci[c];
       if (ii >= 0 && key_less(last_d, last_i, dd, ii) && key_less(dd, ii, bd, bi)) {
@@ -414,19 +494,26 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float *__restrict__ Q
     last_i = bi;
   }
 
-  // Step 2: exact FP32 distances, one candidate at a time across the wave.
+  // Step 2: exact FP32 distances, 8 candidates at a time: lanes 8c .. 8c+7
+  // sum dims part, part + 8, ... of candidate m0 + c, so that 8 row loads
+  // are in flight instead of one.
   float d32 = __builtin_inff();
-  for (int m = 0; m < RR_M; ++m) {
-    const int r = __shfl(sel, m);
-    if (r < 0) break;
-    float s = 0.f;
-    for (int j = lane; j < d; j += 64) {
-      const float t = Qf[(int64_t)q * ldq + j] - Rf[(int64_t)r * ldr + j];
-      s += t * t;
+  {
+    const int grp = lane / 8, part = lane % 8;
+    for (int m0 = 0; m0 < RR_M; m0 += 8) {
+      const int r = __shfl(sel, m0 + grp);
+      float s = 0.f;
+      if (r >= 0)
+        for (int j = part; j < d; j += 8) {
+          const float t = Qf[(int64_t)q * ldq + j] - Rf[(int64_t)r * ldr + j];
+          s += t * t;
+        }
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      s += __shfl_xor(s, 4);
+      const float got = __shfl(s, 8 * ((lane - m0) & 7));
+      if (lane >= m0 && lane < m0 + 8 && sel >= 0) d32 = got;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == m) d32 = s;
   }
   const int my_i = sel >= 0 ? sel : 0x7fffffff;
 
@@ -496,22 +583,23 @@ hipError_t launch_knn_select(const float *dist, int Q, int R, int k, float *out_
 
 size_t knn_bf16_workspace(int Q, int R, int d, int *slices_out) {
   const int dpad = dpad_of(d);
-  const int qblocks = (Q + CQ - 1) / CQ;
-  // Enough reference slices to fill the GPU in whole rounds: the kernel
-  // holds 2 workgroups per CU (its registers), 512 at a time.
-  int slices = (1024 + qblocks - 1) / qblocks;
-  slices = slices < 1 ? 1 : (slices > 8 ? 8 : slices);
+  // 8 reference slices, one per XCD (see knn_pass_kernel), fewer when the
+  // references are fewer than 8 tiles.
+  int slices = 8;
   const int max_slices = (R + CR - 1) / CR;
   if (slices > max_slices) slices = max_slices < 1 ? 1 : max_slices;
   if (slices_out) *slices_out = slices;
-  const size_t ncand = (size_t)slices * 4 * CLIST;
+  const size_t ntiles = ((size_t)R + CR - 1) / CR;
+  auto a256 = [](size_t b) { return (b + 255) / 256 * 256; };
   size_t bytes = 0;
-  bytes += ((size_t)Q * dpad * 4 + 255) / 256 * 256;  // hi | lo BF16
-  bytes += ((size_t)R * dpad * 4 + 255) / 256 * 256;
-  bytes += ((size_t)Q * 4 + 255) / 256 * 256;
-  bytes += ((size_t)R * 4 + 255) / 256 * 256;
-  bytes += ((size_t)Q * ncand * 4 + 255) / 256 * 256;
-  bytes += ((size_t)Q * ncand * 4 + 255) / 256 * 256;
+  bytes += a256((size_t)Q * dpad * 4);  // hi | lo BF16
+  bytes += a256((size_t)R * dpad * 4);
+  bytes += a256((size_t)Q * 4);          // query norms
+  bytes += a256((size_t)R * 4);          // reference norms
+  bytes += a256((size_t)Q * ntiles * 4); // tile minima
+  bytes += a256((size_t)Q * 4);          // tau
+  bytes += a256((size_t)Q * 4);          // candidate counts
+  bytes += 2 * a256((size_t)Q * CAND_CAP * 4);
   return bytes;
 }
 
@@ -521,7 +609,7 @@ hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, in
   int slices = 1;
   knn_bf16_workspace(Q, R, d, &slices);
   const int dpad = dpad_of(d);
-  const int ncand = slices * 4 * CLIST;
+  const int ntiles = (R + CR - 1) / CR;
   char *w = static_cast<char *>(workspace);
   auto take = [&](size_t bytes) {
     char *p = w;
@@ -532,24 +620,40 @@ hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, in
   __bf16 *Rb = reinterpret_cast<__bf16 *>(take((size_t)R * dpad * 4));
   float *qn = reinterpret_cast<float *>(take((size_t)Q * 4));
   float *rn = reinterpret_cast<float *>(take((size_t)R * 4));
-  float *cd = reinterpret_cast<float *>(take((size_t)Q * ncand * 4));
-  int *ci = reinterpret_cast<int *>(take((size_t)Q * ncand * 4));
+  float *tmin = reinterpret_cast<float *>(take((size_t)Q * ntiles * 4));
+  float *tau = reinterpret_cast<float *>(take((size_t)Q * 4));
+  int *cn = reinterpret_cast<int *>(take((size_t)Q * 4));
+  float *cd = reinterpret_cast<float *>(take((size_t)Q * CAND_CAP * 4));
+  int *ci = reinterpret_cast<int *>(take((size_t)Q * CAND_CAP * 4));
 
   prep_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, Q, d, dpad, Qb, qn);
   prep_kernel<<<(R + 3) / 4, 256, 0, s>>>(ref, ldr, R, d, dpad, Rb, rn);
   int slice = (R + slices - 1) / slices;
   slice = (slice + CR - 1) / CR * CR;
-  const dim3 grid((Q + CQ - 1) / CQ, slices);
-  switch (dpad) {
-  case 32: cand_kernel<32><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slice, cd, ci, ncand); break;
-  case 64: cand_kernel<64><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slice, cd, ci, ncand); break;
-  case 96: cand_kernel<96><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slice, cd, ci, ncand); break;
-  case 128: cand_kernel<128><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slice, cd, ci, ncand); break;
-  case 160: cand_kernel<160><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slice, cd, ci, ncand); break;
-  default: return hipErrorInvalidValue;
+  const dim3 grid(((Q + CQ - 1) / CQ) * slices);
+  for (int pass = 1; pass <= 2; ++pass) {
+#define SIMD2_KNN_PASS(D)                                                                      \
+  case D:                                                                                      \
+    if (pass == 1)                                                                             \
+      knn_pass_kernel<D, 1><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slices, slice, tmin,  \
+                                                 ntiles, tau, cd, ci, cn);                     \
+    else                                                                                       \
+      knn_pass_kernel<D, 2><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slices, slice, tmin,  \
+                                                 ntiles, tau, cd, ci, cn);                     \
+    break;
+    switch (dpad) {
+      SIMD2_KNN_PASS(32)
+      SIMD2_KNN_PASS(64)
+      SIMD2_KNN_PASS(96)
+      SIMD2_KNN_PASS(128)
+      SIMD2_KNN_PASS(160)
+    default: return hipErrorInvalidValue;
+    }
+#undef SIMD2_KNN_PASS
+    if (pass == 1) knn_tau_kernel<<<(Q + 3) / 4, 256, 0, s>>>(tmin, ntiles, Q, tau, cn);
   }
-  rerank_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, ref, ldr, Q, d, k, cd, ci, ncand, out_d,
-                                           out_i, do_sqrt, stats);
+  rerank_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, ref, ldr, Q, d, k, cd, ci, cn, CAND_CAP,
+                                           out_d, out_i, do_sqrt, stats);
   return hipGetLastError();
 }
 
