@@ -8,7 +8,11 @@
 // i.e. 32 (i,j,k) terms per instruction.  A B slab of 32 rows (one A word per
 // row of the tile) sits in LDS; A words come straight from global memory.
 // A whole A word that is zero skips its 32 k-steps (wave-uniform test), which
-// pays off on sparse graphs.
+// pays off on sparse graphs.  When the output has too few tiles to fill the
+// GPU (4096 x 4096 bits is 128 tiles), K is split over several workgroups per
+// tile whose partial words are OR-ed into D with atomics (OR is idempotent
+// and commutative, so the result is exact): D is first set to C (or zero) and
+// the change flag is taken by a pass of its own afterwards.
 //
 // int32 kernel (cuasr_orand_srsgemm drop-in): booleans stored one per int.
 // The kernel packs on the fly: X rows (k contiguous) are packed along k with
@@ -16,6 +20,8 @@
 // column, and a thread then ORs the AND of 32-k words per output element
 // (v_and_or_b32 again, 32 terms per instruction).  Its cost is the 4-byte
 // reads of 1-bit data, which is why the packed form exists.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace simd2 {
@@ -42,13 +48,14 @@ __global__ __launch_bounds__(256) void orand_packed_kernel(
     int M, int NW, int K, const uint32_t *__restrict__ A, int64_t lda,
     const uint32_t *__restrict__ B, int64_t ldb, const uint32_t *C,
     int64_t ldc, uint32_t *D, int64_t ldd, const uint32_t *P, int64_t ldp,
-    int *flag, uint32_t last_mask, int tiles_w) {
+    int *flag, uint32_t last_mask, int tiles_w, int ksplit, int kw_per) {
   __shared__ __attribute__((aligned(16))) uint32_t Bs[PK_BK][PK_BW];
   const int tid = tid_x();
   const int tw = tid % PK_NTW;
   const int trr = tid / PK_NTW;
-  const int tile_r = bid_x() / tiles_w;
-  const int tile_w = bid_x() % tiles_w;
+  const int tile = bid_x() / ksplit, split = bid_x() % ksplit;
+  const int tile_r = tile / tiles_w;
+  const int tile_w = tile % tiles_w;
   const int row0 = tile_r * PK_BM + trr * PK_TR;
   const int w0 = tile_w * PK_BW;
 
@@ -59,7 +66,8 @@ __global__ __launch_bounds__(256) void orand_packed_kernel(
     for (int j = 0; j < PK_TW; ++j) acc[i][j] = 0u;
 
   const int nkw = (K + 31) / 32;
-  for (int kw = 0; kw < nkw; ++kw) {
+  const int kw_end = min(nkw, (split + 1) * kw_per);
+  for (int kw = split * kw_per; kw < kw_end; ++kw) {
     // A words of this thread's rows for k in [32 kw, 32 kw + 32).
     uint32_t aw[PK_TR];
     uint32_t any = 0;
@@ -95,6 +103,19 @@ __global__ __launch_bounds__(256) void orand_packed_kernel(
     __syncthreads();
   }
 
+  if (ksplit > 1) {  // partial words into D (set to C or zero beforehand)
+#pragma unroll
+    for (int i = 0; i < PK_TR; ++i) {
+      const int r = row0 + i;
+#pragma unroll
+      for (int j = 0; j < PK_TW; ++j) {
+        const int w = w0 + tw * PK_TW + j;
+        const uint32_t v = acc[i][j] & (w == NW - 1 ? last_mask : ~0u);
+        if (r < M && w < NW && v) atomicOr(&D[(int64_t)r * ldd + w], v);
+      }
+    }
+    return;
+  }
   bool changed = false;
 #pragma unroll
   for (int i = 0; i < PK_TR; ++i) {
@@ -115,6 +136,34 @@ __global__ __launch_bounds__(256) void orand_packed_kernel(
     const bool any = __any(changed);
     if (any && (tid % warpSize) == 0) atomicOr(flag, 1);
   }
+}
+
+// D = C (or zero), last word masked: the start of a K-split product.
+__global__ __launch_bounds__(256) void orand_init_kernel(int M, int NW, const uint32_t *C,
+                                                         int64_t ldc, uint32_t *D, int64_t ldd,
+                                                         uint32_t last_mask) {
+  const int64_t total = (int64_t)M * NW;
+  for (int64_t e = (int64_t)bid_x() * 256 + tid_x(); e < total; e += (int64_t)gridDim.x * 256) {
+    const int r = (int)(e / NW), w = (int)(e % NW);
+    const uint32_t v = C ? C[(int64_t)r * ldc + w] : 0u;
+    D[(int64_t)r * ldd + w] = v & (w == NW - 1 ? last_mask : ~0u);
+  }
+}
+
+// *flag = 1 when D differs from P (last word masked): the change test of a
+// K-split product.
+__global__ __launch_bounds__(256) void orand_diff_kernel(int M, int NW, const uint32_t *D,
+                                                         int64_t ldd, const uint32_t *P,
+                                                         int64_t ldp, uint32_t last_mask,
+                                                         int *flag) {
+  const int64_t total = (int64_t)M * NW;
+  bool changed = false;
+  for (int64_t e = (int64_t)bid_x() * 256 + tid_x(); e < total; e += (int64_t)gridDim.x * 256) {
+    const int r = (int)(e / NW), w = (int)(e % NW);
+    const uint32_t m = w == NW - 1 ? last_mask : ~0u;
+    changed |= (D[(int64_t)r * ldd + w] & m) != (P[(int64_t)r * ldp + w] & m);
+  }
+  if (__any(changed) && tid_x() % 64 == 0) atomicOr(flag, 1);
 }
 
 // ---- int32 booleans -------------------------------------------------------
@@ -221,8 +270,22 @@ hipError_t launch_orand_packed(int M, int N, int K, const uint32_t *A,
   const uint32_t last_mask = (N & 31) ? ((1u << (N & 31)) - 1u) : ~0u;
   const int tiles_r = (M + PK_BM - 1) / PK_BM;
   const int tiles_w = (NW + PK_BW - 1) / PK_BW;
-  orand_packed_kernel<<<dim3(tiles_r * tiles_w), dim3(256), 0, s>>>(
-      M, NW, K, A, lda, B, ldb, C, ldc, D, ldd, P, ldp, flag, last_mask, tiles_w);
+  const int tiles = tiles_r * tiles_w;
+  const int nkw = (K + 31) / 32;
+  // about 1024 workgroups: split K when the tiles alone are fewer
+  int ksplit = std::max(1, std::min({(1024 + tiles - 1) / tiles, 16, nkw}));
+  const int kw_per = (nkw + ksplit - 1) / std::max(ksplit, 1);
+  if (nkw > 0) ksplit = (nkw + kw_per - 1) / kw_per;
+  if (M <= 0 || NW <= 0) return hipSuccess;
+  const int64_t words = (int64_t)M * NW;
+  const int egrid = (int)std::min<int64_t>((words + 255) / 256, 2048);
+  if (ksplit > 1)
+    orand_init_kernel<<<dim3(egrid), dim3(256), 0, s>>>(M, NW, C, ldc, D, ldd, last_mask);
+  orand_packed_kernel<<<dim3(tiles * ksplit), dim3(256), 0, s>>>(
+      M, NW, K, A, lda, B, ldb, C, ldc, D, ldd, P, ldp, flag, last_mask, tiles_w, ksplit,
+      kw_per);
+  if (ksplit > 1 && P && flag)
+    orand_diff_kernel<<<dim3(egrid), dim3(256), 0, s>>>(M, NW, D, ldd, P, ldp, last_mask, flag);
   return hipGetLastError();
 }
 
