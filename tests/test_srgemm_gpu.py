@@ -2,8 +2,9 @@
 
 Bit-exact for the six min/max semirings and or_and (and for plus_mul /
 minus_square where they take the sequential VALU path on shapes off the
-128x128x16 grid); plus_mul on MFMA and the fused minus_square within 1e-5
-relative of FP64 (tolerance stated per test)."""
+128x128x16 grid, and for minus_square on integer-valued data); plus_mul on
+MFMA and the fused minus_square on fractional data within 1e-5 relative of
+FP64 (tolerance stated per test)."""
 import sys
 
 import numpy as np
@@ -103,6 +104,25 @@ def test_plus_mul_and_minus_square_tolerance(gpu, M, N, K):
             ref = fp64_ref(op, A, B, C, epi)
             rel = np.max(np.abs(D - ref) / np.abs(ref))
             assert rel <= 1e-5, (op, rel)
+
+
+def test_minus_square_bit_exact_on_integer_data(gpu, oracle_mod):
+    """On integer-valued inputs (the 4096^3 benchmark's U{1..100}) every
+    (a - b)^2 is an exact integer, so the fused acc + d*d rounds exactly as
+    the oracle's separate product and sum; with the same k order the result
+    is the oracle's bit for bit.  K = 16384 takes the sums past 2^24, where
+    every addition rounds."""
+    rng = np.random.default_rng(13)
+    M, N, K = 64, 96, 16384
+    A = rng.integers(1, 101, (M, K)).astype(np.float32)
+    B = rng.integers(1, 101, (K, N)).astype(np.float32)
+    C = rng.integers(1, 101, (M, N)).astype(np.float32)
+    for epi in (False, True):
+        say(f"minus_square integer {M}x{N}x{K} epi={epi}")
+        D = run(gpu, "minus_square", A, B, C, epi)
+        want = oracle_mod.srgemm_f32("minus_square", A, B, C, epi, threads=8)
+        assert float(want.max()) > 2.0 ** 24
+        assert np.array_equal(bits(D), bits(want))
 
 
 def test_plus_mul_mfma_fractional(gpu):
