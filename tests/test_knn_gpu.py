@@ -69,7 +69,9 @@ def test_knn_mxu_dropin_vs_oracle(oracle_mod):
 
 @pytest.mark.parametrize("Q,R,d,k,clusters", [(2048, 4096, 128, 16, True),
                                               (300, 1000, 50, 5, False),
-                                              (130, 70, 96, 16, False)])
+                                              (130, 70, 96, 16, False),
+                                              (200, 2100, 64, 16, True),
+                                              (100, 40000, 32, 8, True)])
 def test_knn_bf16_vs_fp64(gpu, Q, R, d, k, clusters):
     import simd2_amd.knn as K
     torch = gpu
