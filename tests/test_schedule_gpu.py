@@ -149,3 +149,21 @@ def test_sparse_forms_past_one_column_pass(gpu):
             assert itr2 == itr and torch.equal(Z.view(torch.int32), Y.view(torch.int32)), rep
             assert rep[key] >= n, rep
         del Y, Z
+
+
+@pytest.mark.parametrize("op", ["min_plus", "max_min"])
+def test_exact_check_mode_every_form(gpu, op):
+    """The exact change test (bit comparison) with each squaring form forced:
+    tiles only, row by row, and over the changed entries (directed and
+    symmetric inputs)."""
+    torch = gpu
+    n = 700
+    for kind, X in (("er", er_family(torch, op, n, 4.0 / n, 17)),
+                    ("grid", grid_family(torch, op, n, 18))):
+        Y, itr, st = plain(torch, op, X, True, "exact")
+        assert st == simd2_amd.SUCCESS
+        for rs, cs in ((0, 0), (n, 0), (0, n), (-1, -1)):
+            print(f"{kind} {op} exact row_sparse={rs} changed_sparse={cs}", flush=True)
+            Z, itr2, _, rep = converge_scheduled(op, X.clone(), check="exact", row_sparse=rs,
+                                                 changed_sparse=cs, predicted_itr=3)
+            assert itr2 == itr and torch.equal(Z.view(torch.int32), Y.view(torch.int32)), rep
