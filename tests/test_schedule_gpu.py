@@ -127,8 +127,15 @@ def test_cap_and_edge_sizes(gpu):
     for n in (1, 2, 15, 16, 17, 129):
         X = er_family(torch, "min_plus", n, 0.3, n)
         Y, itr, _ = plain(torch, "min_plus", X, True, "threshold")
-        Z, itr2, _, _ = converge_scheduled("min_plus", X.clone(), predicted_itr=5)
-        assert itr2 == itr and torch.equal(Z, Y), n
+        # automatic forms, then every squaring row by row, then over the
+        # changed entries (X symmetrised so the symmetric form runs too)
+        Xs = torch.minimum(X, X.t())
+        Ys, itrs, _ = plain(torch, "min_plus", Xs, True, "threshold")
+        for Xc, Yc, it, rs, cs in ((X, Y, itr, -1, -1), (X, Y, itr, n, 0), (X, Y, itr, 0, n),
+                                   (Xs, Ys, itrs, 0, n)):
+            Z, itr2, _, _ = converge_scheduled("min_plus", Xc.clone(), predicted_itr=5,
+                                               row_sparse=rs, changed_sparse=cs)
+            assert itr2 == it and torch.equal(Z, Yc), (n, rs, cs)
 
 
 def test_sparse_forms_past_one_column_pass(gpu):
