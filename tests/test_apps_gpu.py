@@ -127,3 +127,21 @@ def test_generator_mode_gtc_and_apsp(gpu, oracle_mod, tmp_path):
     ms, itr, out = run_app(["apsp", str(v), "0.05", "10"], None, tmp_path, v=v)
     Xo, itr_o = oracle_mod.converge_f32("min_plus", g, check="threshold")
     assert itr == itr_o and np.array_equal(out, Xo)
+
+
+def test_report_harness_all_apps(gpu, tmp_path):
+    """python -m simd2_amd.run_apps at a small size: every app runs under both
+    loop structures with the same iteration count and checksum, and the kNN
+    precision form finds knn_mxu's neighbours."""
+    import json
+    from simd2_amd import run_apps
+    out = tmp_path / "apps.json"
+    rows = run_apps.main(["--sizes", "256", "--reps", "1", "--density", "0.02",
+                          "--json", str(out)])
+    assert len(rows) == len(run_apps.APPS) + 1
+    assert json.loads(out.read_text()) == rows
+    for r in rows:
+        if r["app"] == "PLD":
+            assert r["same_neighbours"] >= 0.99, r
+        else:
+            assert r["same_result"] and r["iterations"] >= 1, r
