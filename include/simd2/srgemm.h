@@ -109,7 +109,14 @@ int cuasr_orand_srsgemm(int M, int N, int K, int const *A, int lda,
                         int const *B, int ldb, int *C, int ldc, int *D,
                         bool do_epilogue_min, hipStream_t stream SIMD2_DEFAULT_STREAM);
 /* replaces cuasr_plusmiussquare_srsgemm (the reference's spelling),
- * kernel/srgemm.cuh:552-626: D = [C +] sum_k (A(i,k) - B(k,j))^2. */
+ * kernel/srgemm.cuh:552-626: D = [C +] sum_k (A(i,k) - B(k,j))^2.
+ * Each 64x64 tile of D whose rows of A and columns of B hold only integers
+ * in [-128, 127] and whose sums are at most 2^24 is computed on the int8
+ * matrix cores as |a|^2 + |b|^2 - 2 a.b, exactly; for such a tile the FP32
+ * sum k by k of host_srgemm is that same integer, so D is bit-identical.
+ * Other tiles run the direct FP32 kernel.  The int8 path keeps one scratch
+ * buffer per (device, stream), about (M + N) * K bytes, until
+ * simd2_release_scratch; it is not used while the stream is being captured. */
 int cuasr_plusmiussquare_srsgemm(int M, int N, int K, float const *A, int lda,
                                  float const *B, int ldb, float *C, int ldc,
                                  float *D, bool do_epilogue_min,
@@ -132,7 +139,8 @@ int simd2_srgemm(int op, int M, int N, int K, const void *A, int lda,
  * time, kernel/srgemm.cuh:574-583): -1 = automatic, 0 = 128x128 tiles (8x8
  * per thread), 1 = 64x64 (4x4), 2 = 64x128 (4x8 per thread; 64 along M and
  * 128 along N of the column-major call).  Used by the autotuner and benchmarks; results are
- * identical for every configuration. */
+ * identical for every configuration.  For minus_square an explicit
+ * configuration runs the direct FP32 kernel on every tile (no int8 path). */
 int simd2_srgemm_ex(int op, int M, int N, int K, const void *A, int lda, const void *B,
                     int ldb, const void *C, int ldc, void *D, int ldd, int do_epilogue,
                     int tile_config, hipStream_t stream);
@@ -160,6 +168,17 @@ int simd2_orand_packed(int M, int N, int K, const uint32_t *A, int lda_words,
                        const uint32_t *B, int ldb_words, const uint32_t *C,
                        int ldc_words, uint32_t *D, int ldd_words,
                        int do_epilogue, hipStream_t stream);
+
+/* 64x64 tiles of the last minus_square call on `stream` (current device)
+ * computed on the int8 matrix cores and on the direct FP32 kernel (no
+ * counterpart in the reference; for tests and reports).  Synchronises
+ * `stream`. */
+int simd2_minus_square_tiles(hipStream_t stream, int *mfma_tiles, int *valu_tiles);
+
+/* Frees the scratch buffers the minus_square int8 path keeps per (device,
+ * stream), after synchronising each device that holds one (no counterpart in
+ * the reference). */
+int simd2_release_scratch(void);
 
 /* Library / device information for reporting (clock, CU count). */
 int simd2_device_info(int *cu_count, int *clock_khz, char *name, int name_len);

@@ -120,6 +120,10 @@ def lib() -> ctypes.CDLL:
         L.simd2_status_string.restype = ctypes.c_char_p
         L.simd2_device_info.argtypes = [ctypes.POINTER(i), ctypes.POINTER(i), ctypes.c_char_p, i]
         L.simd2_device_info.restype = i
+        L.simd2_minus_square_tiles.argtypes = [p, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.simd2_minus_square_tiles.restype = i
+        L.simd2_release_scratch.argtypes = []
+        L.simd2_release_scratch.restype = i
         for name in ("comp_update", "comp_update2", "comp_update_int"):
             fn = getattr(L, name)
             fn.argtypes = [p, p, p, p, i, i]
@@ -338,6 +342,20 @@ def h2f(x):
     out = torch.empty(x.shape, dtype=torch.float32, device=x.device)
     lib().h2f_device(_ptr(out), _ptr(x), x.numel())
     return out
+
+
+def minus_square_tiles(stream=None):
+    """(64x64 tiles on the int8 matrix cores, tiles on the direct FP32 kernel)
+    of the last minus_square call on ``stream`` (simd2_minus_square_tiles)."""
+    mf, va = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().simd2_minus_square_tiles(_stream(stream), ctypes.byref(mf), ctypes.byref(va)),
+           "simd2_minus_square_tiles")
+    return mf.value, va.value
+
+
+def release_scratch():
+    """Free the minus_square int8 path's scratch buffers (simd2_release_scratch)."""
+    _check(lib().simd2_release_scratch(), "simd2_release_scratch")
 
 
 def device_info():

@@ -64,6 +64,8 @@ int srgemm_f32_cm(int op, int M, int N, int K, const float *A, int lda, const fl
   hipError_t e;
   if (op == SIMD2_PLUS_MUL) {
     e = launch_plusmul_mfma(N, M, K, B, ldb, A, lda, D, ldd, epi ? C : nullptr, ldc, s);
+  } else if (op == SIMD2_MINUS_SQUARE && cfg == CFG_AUTO) {
+    e = launch_minus_square(N, M, K, B, ldb, A, lda, D, ldd, epi ? C : nullptr, ldc, s);
   } else {
     Epilogue ep;
     if (epi) {
@@ -217,6 +219,13 @@ int simd2_orand_packed(int M, int N, int K, const uint32_t *A, int lda_words,
                                         do_epilogue ? C : nullptr, ldc_words, D, ldd_words,
                                         nullptr, 0, nullptr, stream));
 }
+
+int simd2_minus_square_tiles(hipStream_t stream, int *mfma_tiles, int *valu_tiles) {
+  if (!mfma_tiles || !valu_tiles) return SIMD2_ERROR_NULL_POINTER;
+  return hip_status(minus_square_tile_counts(stream, mfma_tiles, valu_tiles));
+}
+
+int simd2_release_scratch(void) { return hip_status(minus_square_release_scratch()); }
 
 int simd2_device_info(int *cu_count, int *clock_khz, char *name, int name_len) {
   int dev = 0;

@@ -24,6 +24,8 @@ struct Epilogue {
   int *tile_flags = nullptr; // per output tile: 1 changed / 0 unchanged
   int mode = 0;              // change test: 0 bits differ, 1 |p-o| > 1e-4,
                              // 2 APLP: o in (-1,1) -> -FLT_MAX, then o != p
+  const int *gate = nullptr; // when set, tile (tr, tc) is computed only if
+  int gate_ld = 0;           //   gate[tr * gate_ld + tc] != 0
 };
 
 enum CheckMode { CHECK_EXACT = 0, CHECK_THRESHOLD = 1, CHECK_APLP = 2 };
@@ -39,6 +41,18 @@ hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,
                            int64_t ldx, const float *Y, int64_t ldy, float *O,
                            int64_t ldo, const Epilogue &e, hipStream_t s,
                            int cfg = CFG_AUTO);
+
+// minus_square, same row-major problem, with C added when set: tiles whose
+// operands are integers in [-128, 127] and whose sums stay <= 2^24 on the
+// int8 matrix cores (exact), the rest on the VALU kernel (msq_exact.hip).
+hipError_t launch_minus_square(int R, int Cn, int K, const float *X, int64_t ldx,
+                               const float *Y, int64_t ldy, float *O, int64_t ldo,
+                               const float *C, int64_t ldc, hipStream_t s);
+// 64x64 tiles of the last launch_minus_square on stream s (of the current
+// device) computed on the matrix cores / on the VALU kernel; synchronises s.
+hipError_t minus_square_tile_counts(hipStream_t s, int *mfma_tiles, int *valu_tiles);
+// Frees the scratch buffers of launch_minus_square (every device).
+hipError_t minus_square_release_scratch();
 
 // plus_mul on MFMA (v_mfma_f32_16x16x4_f32), same row-major problem.
 hipError_t launch_plusmul_mfma(int R, int Cn, int K, const float *X,

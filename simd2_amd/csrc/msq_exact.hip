@@ -1,0 +1,425 @@
+// minus_square on the int8 matrix cores, exact, for operands that hold small
+// integers.
+//
+// D = [C +] sum_k (X[r][k] - Y[k][c])^2 is computed as
+//   ||x_r||^2 + ||y_c||^2 - 2 x_r . y_c
+// with the dot products on v_mfma_i32_16x16x64_i8 (int8 in, int32
+// accumulate) and the norms in int32.  When every X and Y entry is an integer
+// in [-128, 127] every term is exact, so this is the exact integer S.  The
+// sequential FP32 sum of the oracle (kernel/host_srgemm.h:34-48: acc = 0,
+// acc += (a-b)^2 for k ascending) reaches S through partial sums that are
+// integers no larger than S (the terms are >= 0); when S <= 2^24 each of them
+// is a float, so the FP32 sum is S bit for bit and D = fl(S [+ C]) equals the
+// oracle's.  A 64x64 output tile is written here only when all of its rows
+// and columns passed the integer check and all of its sums are <= 2^24;
+// otherwise its gate entry asks the VALU kernel to compute it the direct way.
+//
+// Steps (one stream, no host synchronisation):
+//   1. quantize: X rows and Y columns to int8, k contiguous and zero-padded
+//      to a multiple of 64, with each row's / column's exact sum of squares
+//      (bit 31 set: some entry was not an integer in [-128, 127]);
+//   2. MFMA tiles 128x128 (4 waves x 64x64 quadrants of 4x4 16x16 blocks),
+//      64-deep int8 slabs double-buffered in LDS, epilogue S -> float [+ C];
+//   3. the VALU kernel on the 64x64 grid computes the tiles whose gate is 1
+//      (srgemm_valu.hip; every other workgroup returns at once).
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace simd2 {
+
+namespace {
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int QT = 128;        // MFMA output tile
+constexpr int QK = 64;         // k per slab = one MFMA k-step
+constexpr int QLD = QK + 16;   // LDS row stride (bytes): 16 rows of a fragment read in distinct banks
+constexpr int QR = 32;         // rows (or columns) per quantize workgroup
+constexpr int QS = 256;        // k per quantize step
+constexpr int QSLD = QS + 16;
+constexpr int EXACT_LIMIT = 1 << 24;
+constexpr int NUM_XCD = 8;
+
+__device__ __forceinline__ bool small_int(float v) {
+  return v == rintf(v) && v >= -128.0f && v <= 127.0f;
+}
+
+// Grid: Rq/32 workgroups for X rows, then Cq/32 for Y columns.  Rows in
+// [n, Rq) are written as zeros; norms only for rows < n.
+__global__ __launch_bounds__(256) void msq_quantize_kernel(
+    int R, int Cn, int K, int Kp, const float *__restrict__ X, int64_t ldx,
+    const float *__restrict__ Y, int64_t ldy, int8_t *__restrict__ Xq,
+    int8_t *__restrict__ Yq, int *__restrict__ nx, int *__restrict__ ny, int bx) {
+  __shared__ __attribute__((aligned(16))) int8_t T[QR][QSLD];
+  __shared__ int rowbad[QR];
+  const int tid = tid_x();
+  const bool isx = bid_x() < bx;
+  const int i0 = (isx ? bid_x() : bid_x() - bx) * QR;
+  const int n = isx ? R : Cn;
+  int8_t *Q = isx ? Xq : Yq;
+  if (tid < QR) rowbad[tid] = 0;
+  __syncthreads();
+  // write phase: row wi, bytes [wb, wb + 32) of the step
+  const int wi = tid / 8, wb = (tid % 8) * 32;
+  int norm = 0;
+  for (int k0 = 0; k0 < Kp; k0 += QS) {
+    if (isx) {
+      // X[i][k], k contiguous: a wave reads 64 consecutive k of one row
+      const int kk = tid % 64, rg = tid / 64;
+#pragma unroll
+      for (int j = 0; j < QR / 4; ++j) {
+        const int ri = rg + 4 * j;
+        const int i = i0 + ri;
+#pragma unroll
+        for (int s = 0; s < QS / 64; ++s) {
+          const int k = k0 + 64 * s + kk;
+          const float v = (i < n && k < K) ? X[(int64_t)i * ldx + k] : 0.0f;
+          const bool ok = small_int(v);
+          if (!ok) rowbad[ri] = 1;
+          T[ri][64 * s + kk] = ok ? (int8_t)(int)v : (int8_t)0;
+        }
+      }
+    } else {
+      // Y[k][c], c contiguous: half a wave reads 32 consecutive columns
+      const int cc = tid % QR, kr = tid / QR;
+      const int c = i0 + cc;
+#pragma unroll
+      for (int j = 0; j < QS / 8; ++j) {
+        const int k = k0 + kr + 8 * j;
+        const float v = (c < n && k < K) ? Y[(int64_t)k * ldy + c] : 0.0f;
+        const bool ok = small_int(v);
+        if (!ok) rowbad[cc] = 1;
+        T[cc][kr + 8 * j] = ok ? (int8_t)(int)v : (int8_t)0;
+      }
+    }
+    __syncthreads();
+    const uint4 w0 = *reinterpret_cast<const uint4 *>(&T[wi][wb]);
+    const uint4 w1 = *reinterpret_cast<const uint4 *>(&T[wi][wb + 16]);
+    const uint32_t words[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int v = (int)(int8_t)(words[q] >> (8 * b));
+        norm += v * v;
+      }
+    }
+    // Kp is a multiple of 64 (not of the 256 of a step): a 32-byte piece is
+    // either wholly inside the row or wholly past it.
+    if (k0 + wb < Kp) {
+      uint4 *dst = reinterpret_cast<uint4 *>(Q + (int64_t)(i0 + wi) * Kp + k0 + wb);
+      dst[0] = w0;
+      dst[1] = w1;
+    }
+    __syncthreads();
+  }
+  norm += __shfl_xor(norm, 1);
+  norm += __shfl_xor(norm, 2);
+  norm += __shfl_xor(norm, 4);
+  const int i = i0 + wi;
+  if (tid % 8 == 0 && i < n) {
+    const uint32_t out = (uint32_t)norm | (rowbad[wi] ? 0x80000000u : 0u);
+    (isx ? nx : ny)[i] = (int)out;
+  }
+}
+
+__device__ __forceinline__ void map_tile(int b, int tiles_r, int tiles_c, int &tr, int &tc) {
+  const int nt = tiles_r * tiles_c;
+  int t = b;
+  if (nt % NUM_XCD == 0) t = (b % NUM_XCD) * (nt / NUM_XCD) + b / NUM_XCD;
+  constexpr int GROUP = 8;
+  const int width = GROUP * tiles_c;
+  const int group = t / width;
+  const int first = group * GROUP;
+  const int gsz = min(tiles_r - first, GROUP);
+  const int in = t - group * width;
+  tr = first + in % gsz;
+  tc = in / gsz;
+}
+
+// Fragment layout of v_mfma_i32_16x16x64_i8: lane l supplies 16 bytes of
+// row l % 16 of the A block and 16 bytes of column l % 16 of the B block,
+// both from k group l / 16; accumulator element v of lane l is block
+// element (4 * (l / 16) + v, l % 16).  A and B take their k bytes in the same
+// order, and integer sums are exact, so the order of k within a group does
+// not change the result.
+__global__ __launch_bounds__(256) void msq_mfma_kernel(
+    int R, int Cn, int Kp, const int8_t *__restrict__ Xq, const int8_t *__restrict__ Yq,
+    const int *__restrict__ nx, const int *__restrict__ ny, float *O, int64_t ldo,
+    const float *C, int64_t ldc, int *__restrict__ gate, int gate_ld, int tiles_r,
+    int tiles_c) {
+  __shared__ __attribute__((aligned(16))) int8_t Xs[2][QT * QLD];
+  __shared__ __attribute__((aligned(16))) int8_t Ys[2][QT * QLD];
+  __shared__ int qbad[4];  // rows 0-63, rows 64-127, columns 0-63, columns 64-127
+  const int tid = tid_x();
+  const int lane = tid % 64;
+  const int wave = tid / 64;
+  const int wr = wave / 2, wc = wave % 2;
+  int tr, tc;
+  map_tile(bid_x(), tiles_r, tiles_c, tr, tc);
+  const int row0 = tr * QT, col0 = tc * QT;
+
+  // Quadrants with a row or column that failed the integer check go to the
+  // VALU kernel; when all four do, skip the products.
+  if (tid < 4) qbad[tid] = 0;
+  __syncthreads();
+  {
+    const int t = tid % QT;
+    if (tid < QT) {
+      const int r = row0 + t;
+      if (r < R && nx[r] < 0) qbad[t / 64] = 1;
+    } else {
+      const int c = col0 + t;
+      if (c < Cn && ny[c] < 0) qbad[2 + t / 64] = 1;
+    }
+  }
+  __syncthreads();
+  int *g = gate + (int64_t)(tr * 2 + wr) * gate_ld + tc * 2 + wc;
+  const bool mybad = qbad[wr] | qbad[2 + wc];
+  if ((qbad[0] | qbad[2]) & (qbad[0] | qbad[3]) & (qbad[1] | qbad[2]) & (qbad[1] | qbad[3])) {
+    if (lane == 0) *g = 1;
+    return;
+  }
+
+  const int sr = tid / 4, sc = (tid % 4) * 16;
+  const int8_t *xp = Xq + (int64_t)(row0 + sr) * Kp + sc;
+  const int8_t *yp = Yq + (int64_t)(col0 + sr) * Kp + sc;
+  const int64_t half = (int64_t)64 * Kp;
+  uint4 x0, x1, y0, y1;
+  auto load = [&]() {
+    x0 = *reinterpret_cast<const uint4 *>(xp);
+    x1 = *reinterpret_cast<const uint4 *>(xp + half);
+    y0 = *reinterpret_cast<const uint4 *>(yp);
+    y1 = *reinterpret_cast<const uint4 *>(yp + half);
+  };
+  auto stage = [&](int b) {
+    *reinterpret_cast<uint4 *>(&Xs[b][sr * QLD + sc]) = x0;
+    *reinterpret_cast<uint4 *>(&Xs[b][(sr + 64) * QLD + sc]) = x1;
+    *reinterpret_cast<uint4 *>(&Ys[b][sr * QLD + sc]) = y0;
+    *reinterpret_cast<uint4 *>(&Ys[b][(sr + 64) * QLD + sc]) = y1;
+  };
+
+  i32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
+
+  const int nks = Kp / QK;
+  load();
+  stage(0);
+  __syncthreads();
+  const int fr = lane % 16, fk = (lane / 16) * 16;
+  for (int ks = 0; ks < nks; ++ks) {
+    const int buf = ks & 1;
+    const bool more = ks + 1 < nks;
+    if (more) {
+      xp += QK;
+      yp += QK;
+      load();
+    }
+    i32x4 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      a[i] = *reinterpret_cast<const i32x4 *>(&Xs[buf][(wr * 64 + i * 16 + fr) * QLD + fk]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      b[j] = *reinterpret_cast<const i32x4 *>(&Ys[buf][(wc * 64 + j * 16 + fr) * QLD + fk]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) stage(buf ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue of this wave's 64x64 quadrant: first whether every sum is
+  // exact in FP32, then the stores.
+  const int qr0 = row0 + wr * 64, qc0 = col0 + wc * 64;
+  int nyv[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int c = qc0 + j * 16 + fr;
+    nyv[j] = c < Cn ? ny[c] : 0;
+  }
+  bool bad = mybad;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = qr0 + i * 16 + (lane / 16) * 4 + v;
+      const int nxv = r < R ? nx[r] : 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = qc0 + j * 16 + fr;
+        const int64_t s = (int64_t)nxv + nyv[j] - 2 * (int64_t)acc[i][j][v];
+        if (r < R && c < Cn && s > EXACT_LIMIT) bad = true;
+      }
+    }
+  if (__ballot(bad) != 0) {
+    if (lane == 0) *g = 1;
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = qr0 + i * 16 + (lane / 16) * 4 + v;
+      if (r >= R) continue;
+      const int nxv = nx[r];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = qc0 + j * 16 + fr;
+        if (c >= Cn) continue;
+        // 0 <= s <= 2^24 here: the conversion is exact
+        float d = (float)(nxv + nyv[j] - 2 * acc[i][j][v]);
+        if (C) d = d + C[(int64_t)r * ldc + c];
+        O[(int64_t)r * ldo + c] = d;
+      }
+    }
+  if (lane == 0) *g = 0;
+}
+
+// Scratch buffers of the exact path, one per (device, stream), grown on
+// demand.  The whole launch sequence runs under the lock, so a buffer is
+// never freed between the time a call takes it and the time its kernels are
+// enqueued; growth synchronises the stream before freeing the old buffer.
+struct Scratch {
+  int dev;
+  hipStream_t s;
+  void *p;
+  size_t bytes;
+  // gate array of the last call, for the statistics
+  const int *gate;
+  int gate_ld, tiles_r, tiles_c;
+};
+std::mutex g_scratch_mu;
+std::vector<Scratch> g_scratch;
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+Scratch *scratch_for(int dev, hipStream_t s, size_t bytes) {
+  for (auto &e : g_scratch)
+    if (e.dev == dev && e.s == s) {
+      if (e.bytes >= bytes) return &e;
+      if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
+      (void)hipFree(e.p);
+      e.p = nullptr;
+      e.bytes = 0;
+      e.gate = nullptr;
+      if (hipMalloc(&e.p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        e.p = nullptr;
+        return nullptr;
+      }
+      e.bytes = bytes;
+      return &e;
+    }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  g_scratch.push_back({dev, s, p, bytes, nullptr, 0, 0, 0});
+  return &g_scratch.back();
+}
+
+}  // namespace
+
+hipError_t launch_minus_square(int R, int Cn, int K, const float *X, int64_t ldx,
+                               const float *Y, int64_t ldy, float *O, int64_t ldo,
+                               const float *C, int64_t ldc, hipStream_t s) {
+  Epilogue ep;
+  ep.C = C;
+  ep.ldc = ldc;
+  // K <= 65536 keeps every norm below 2^30 and every int32 dot product in
+  // range.
+  if (R <= 0 || Cn <= 0 || K <= 0 || K > 65536)
+    return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
+  // Under graph capture the scratch buffer could be freed while a graph
+  // still refers to it: take the direct path there.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const int Kp = (K + QK - 1) / QK * QK;
+  const int Rq = (R + QT - 1) / QT * QT, Cq = (Cn + QT - 1) / QT * QT;
+  const int gate_ld = Cq / 64, gate_n = (Rq / 64) * gate_ld;
+  const size_t bx = align256((size_t)Rq * Kp), by = align256((size_t)Cq * Kp);
+  const size_t bnx = align256((size_t)R * 4), bny = align256((size_t)Cn * 4);
+  const size_t bytes = bx + by + bnx + bny + align256((size_t)gate_n * 4);
+
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  Scratch *sc = scratch_for(dev, s, bytes);
+  if (!sc) return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
+  char *base = static_cast<char *>(sc->p);
+  int8_t *Xq = reinterpret_cast<int8_t *>(base);
+  int8_t *Yq = reinterpret_cast<int8_t *>(base + bx);
+  int *nx = reinterpret_cast<int *>(base + bx + by);
+  int *ny = reinterpret_cast<int *>(base + bx + by + bnx);
+  int *gate = reinterpret_cast<int *>(base + bx + by + bnx + bny);
+
+  const int qbx = Rq / QR, qby = Cq / QR;
+  msq_quantize_kernel<<<qbx + qby, 256, 0, s>>>(R, Cn, K, Kp, X, ldx, Y, ldy, Xq, Yq, nx, ny,
+                                                qbx);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int tiles_r = Rq / QT, tiles_c = Cq / QT;
+  msq_mfma_kernel<<<tiles_r * tiles_c, 256, 0, s>>>(R, Cn, Kp, Xq, Yq, nx, ny, O, ldo, C, ldc,
+                                                    gate, gate_ld, tiles_r, tiles_c);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  sc->gate = gate;
+  sc->gate_ld = gate_ld;
+  sc->tiles_r = (R + 63) / 64;
+  sc->tiles_c = (Cn + 63) / 64;
+  ep.gate = gate;
+  ep.gate_ld = gate_ld;
+  return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s, CFG_S64);
+}
+
+hipError_t minus_square_tile_counts(hipStream_t s, int *mfma_tiles, int *valu_tiles) {
+  *mfma_tiles = 0;
+  *valu_tiles = 0;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  for (auto &sc : g_scratch)
+    if (sc.dev == dev && sc.s == s && sc.gate) {
+      std::vector<int> g((size_t)sc.tiles_r * sc.gate_ld);
+      e = hipStreamSynchronize(s);
+      if (e == hipSuccess)
+        e = hipMemcpy(g.data(), sc.gate, g.size() * sizeof(int), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return e;
+      for (int r = 0; r < sc.tiles_r; ++r)
+        for (int c = 0; c < sc.tiles_c; ++c) ++*(g[(size_t)r * sc.gate_ld + c] ? valu_tiles
+                                                                              : mfma_tiles);
+      return hipSuccess;
+    }
+  return hipSuccess;
+}
+
+hipError_t minus_square_release_scratch() {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  int cur = 0;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  for (auto &sc : g_scratch) {
+    if (hipSetDevice(sc.dev) == hipSuccess && hipDeviceSynchronize() == hipSuccess)
+      (void)hipFree(sc.p);
+  }
+  g_scratch.clear();
+  return hipSetDevice(cur);
+}
+
+}  // namespace simd2

@@ -37,6 +37,7 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
   __shared__ ValuSmem<BM, BN, BK> sm;
   int tr, tc;
   map_tile(bid_x(), tiles_r, tiles_c, tr, tc);
+  if (e.gate && e.gate[tr * e.gate_ld + tc] == 0) return;
   valu_tile<Op, BM, BN, TM, TN, BK, CHECKED>(
       R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tr * BM, tc * BN,
       e.tile_flags ? &e.tile_flags[tr * tiles_c + tc] : nullptr, sm);
