@@ -15,9 +15,10 @@
 // otherwise its gate entry asks the VALU kernel to compute it the direct way.
 //
 // Steps (one stream, no host synchronisation):
-//   1. quantize: X rows and Y columns to int8, k contiguous and zero-padded
-//      to a multiple of 64, with each row's / column's exact sum of squares
-//      (bit 31 set: some entry was not an integer in [-128, 127]);
+//   1. quantize: X rows and Y columns to int8 in fragment order (below),
+//      zero-padded to 128 rows / columns and to a multiple of 64 in k, with
+//      each row's / column's exact sum of squares (bit 31 set: some entry was
+//      not an integer in [-128, 127]);
 //   2. MFMA tiles 128x128 (4 waves x 64x64 quadrants of 4x4 16x16 blocks),
 //      64-deep int8 slabs double-buffered in LDS, epilogue S -> float [+ C];
 //   3. the VALU kernel on the 64x64 grid computes the tiles whose gate is 1
@@ -37,19 +38,33 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int QT = 128;        // MFMA output tile
 constexpr int QK = 64;         // k per slab = one MFMA k-step
-constexpr int QLD = QK + 16;   // LDS row stride (bytes): 16 rows of a fragment read in distinct banks
+constexpr int QSLAB = QT * QK; // bytes of one 128-row panel slab (8 fragment blocks)
 constexpr int QR = 32;         // rows (or columns) per quantize workgroup
 constexpr int QS = 256;        // k per quantize step
 constexpr int QSLD = QS + 16;
 constexpr int EXACT_LIMIT = 1 << 24;
 constexpr int NUM_XCD = 8;
 
+// Fragment order of the int8 operands.  A 16 x 64 block (16 rows or columns
+// by 64 k) is 1 KB stored as the 64 lanes of v_mfma_i32_16x16x64_i8 read it:
+// lane l = (row % 16) + 16 * (k % 64 / 16) holds bytes k % 16 = 0..15.  The
+// eight blocks of a 128-row panel for one 64-k slab are consecutive (8 KB),
+// and the slabs of a panel follow each other.  One slab of a workgroup tile
+// is then one contiguous 8 KB run per operand, and global loads, LDS stores
+// and fragment reads are all 16 bytes per lane at consecutive addresses.
+__device__ __forceinline__ int64_t frag_offset(int row, int k, int Kp) {
+  return ((int64_t)(row / QT) * (Kp / QK) + k / QK) * QSLAB + (row % QT / 16) * 1024 +
+         ((row % 16) + 16 * (k % QK / 16)) * 16 + k % 16;
+}
+
 __device__ __forceinline__ bool small_int(float v) {
   return v == rintf(v) && v >= -128.0f && v <= 127.0f;
 }
 
 // Grid: Rq/32 workgroups for X rows, then Cq/32 for Y columns.  Rows in
-// [n, Rq) are written as zeros; norms only for rows < n.
+// [n, Rq) are written as zeros; norms only for rows < n.  Every load of a
+// step is issued before any is used (addresses clamped into the matrix,
+// values outside it replaced by 0), so a step waits for memory once.
 __global__ __launch_bounds__(256) void msq_quantize_kernel(
     int R, int Cn, int K, int Kp, const float *__restrict__ X, int64_t ldx,
     const float *__restrict__ Y, int64_t ldy, int8_t *__restrict__ Xq,
@@ -62,38 +77,54 @@ __global__ __launch_bounds__(256) void msq_quantize_kernel(
   const int n = isx ? R : Cn;
   int8_t *Q = isx ? Xq : Yq;
   if (tid < QR) rowbad[tid] = 0;
-  __syncthreads();
   // write phase: row wi, bytes [wb, wb + 32) of the step
   const int wi = tid / 8, wb = (tid % 8) * 32;
   int norm = 0;
-  for (int k0 = 0; k0 < Kp; k0 += QS) {
+  uint32_t badmask = 0;  // X: bit j = row rg + 4j; Y: bit 0 = column cc
+  constexpr int NV = QR * QS / 256;
+  auto issue = [&](int k0, float (&v)[NV]) {
     if (isx) {
       // X[i][k], k contiguous: a wave reads 64 consecutive k of one row
       const int kk = tid % 64, rg = tid / 64;
 #pragma unroll
       for (int j = 0; j < QR / 4; ++j) {
-        const int ri = rg + 4 * j;
-        const int i = i0 + ri;
+        const int i = min(i0 + rg + 4 * j, n - 1);
 #pragma unroll
-        for (int s = 0; s < QS / 64; ++s) {
-          const int k = k0 + 64 * s + kk;
-          const float v = (i < n && k < K) ? X[(int64_t)i * ldx + k] : 0.0f;
-          const bool ok = small_int(v);
-          if (!ok) rowbad[ri] = 1;
-          T[ri][64 * s + kk] = ok ? (int8_t)(int)v : (int8_t)0;
-        }
+        for (int s = 0; s < QS / 64; ++s)
+          v[j * (QS / 64) + s] = X[(int64_t)i * ldx + min(k0 + 64 * s + kk, K - 1)];
       }
     } else {
       // Y[k][c], c contiguous: half a wave reads 32 consecutive columns
       const int cc = tid % QR, kr = tid / QR;
-      const int c = i0 + cc;
+      const int c = min(i0 + cc, n - 1);
 #pragma unroll
-      for (int j = 0; j < QS / 8; ++j) {
+      for (int j = 0; j < NV; ++j) v[j] = Y[(int64_t)min(k0 + kr + 8 * j, K - 1) * ldy + c];
+    }
+  };
+  auto consume = [&](int k0, const float (&v)[NV]) {
+    if (isx) {
+      const int kk = tid % 64, rg = tid / 64;
+#pragma unroll
+      for (int j = 0; j < QR / 4; ++j) {
+        const int ri = rg + 4 * j;
+#pragma unroll
+        for (int s = 0; s < QS / 64; ++s) {
+          const int k = k0 + 64 * s + kk;
+          const float x = (i0 + ri < n && k < K) ? v[j * (QS / 64) + s] : 0.0f;
+          const bool ok = small_int(x);
+          badmask |= ok ? 0u : 1u << j;
+          T[ri][64 * s + kk] = ok ? (int8_t)(int)x : (int8_t)0;
+        }
+      }
+    } else {
+      const int cc = tid % QR, kr = tid / QR;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
         const int k = k0 + kr + 8 * j;
-        const float v = (c < n && k < K) ? Y[(int64_t)k * ldy + c] : 0.0f;
-        const bool ok = small_int(v);
-        if (!ok) rowbad[cc] = 1;
-        T[cc][kr + 8 * j] = ok ? (int8_t)(int)v : (int8_t)0;
+        const float y = (i0 + cc < n && k < K) ? v[j] : 0.0f;
+        const bool ok = small_int(y);
+        badmask |= ok ? 0u : 1u;
+        T[cc][kr + 8 * j] = ok ? (int8_t)(int)y : (int8_t)0;
       }
     }
     __syncthreads();
@@ -104,19 +135,40 @@ __global__ __launch_bounds__(256) void msq_quantize_kernel(
     for (int q = 0; q < 8; ++q) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const int v = (int)(int8_t)(words[q] >> (8 * b));
-        norm += v * v;
+        const int e = (int)(int8_t)(words[q] >> (8 * b));
+        norm += e * e;
       }
     }
     // Kp is a multiple of 64 (not of the 256 of a step): a 32-byte piece is
     // either wholly inside the row or wholly past it.
     if (k0 + wb < Kp) {
-      uint4 *dst = reinterpret_cast<uint4 *>(Q + (int64_t)(i0 + wi) * Kp + k0 + wb);
-      dst[0] = w0;
-      dst[1] = w1;
+      *reinterpret_cast<uint4 *>(Q + frag_offset(i0 + wi, k0 + wb, Kp)) = w0;
+      *reinterpret_cast<uint4 *>(Q + frag_offset(i0 + wi, k0 + wb + 16, Kp)) = w1;
     }
     __syncthreads();
+  };
+  // Two register sets: the loads of the next step are in flight while this
+  // step is converted and stored.
+  float va[NV], vb[NV];
+  issue(0, va);
+  for (int k0 = 0; k0 < Kp; k0 += 2 * QS) {
+    if (k0 + QS < Kp) issue(k0 + QS, vb);
+    consume(k0, va);
+    if (k0 + QS < Kp) {
+      if (k0 + 2 * QS < Kp) issue(k0 + 2 * QS, va);
+      consume(k0 + QS, vb);
+    }
   }
+  if (badmask) {
+    if (isx) {
+#pragma unroll
+      for (int j = 0; j < QR / 4; ++j)
+        if (badmask & (1u << j)) rowbad[tid / 64 + 4 * j] = 1;
+    } else {
+      rowbad[tid % QR] = 1;
+    }
+  }
+  __syncthreads();
   norm += __shfl_xor(norm, 1);
   norm += __shfl_xor(norm, 2);
   norm += __shfl_xor(norm, 4);
@@ -147,14 +199,16 @@ __device__ __forceinline__ void map_tile(int b, int tiles_r, int tiles_c, int &t
 // element (4 * (l / 16) + v, l % 16).  A and B take their k bytes in the same
 // order, and integer sums are exact, so the order of k within a group does
 // not change the result.
-__global__ __launch_bounds__(256) void msq_mfma_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void msq_mfma_kernel(
     int R, int Cn, int Kp, const int8_t *__restrict__ Xq, const int8_t *__restrict__ Yq,
     const int *__restrict__ nx, const int *__restrict__ ny, float *O, int64_t ldo,
     const float *C, int64_t ldc, int *__restrict__ gate, int gate_ld, int tiles_r,
     int tiles_c) {
-  __shared__ __attribute__((aligned(16))) int8_t Xs[2][QT * QLD];
-  __shared__ __attribute__((aligned(16))) int8_t Ys[2][QT * QLD];
-  __shared__ int qbad[4];  // rows 0-63, rows 64-127, columns 0-63, columns 64-127
+  __shared__ __attribute__((aligned(16))) int8_t Xs[2][QSLAB];
+  __shared__ __attribute__((aligned(16))) int8_t Ys[2][QSLAB];
+  // Quadrant flags (rows 0-63, rows 64-127, columns 0-63, columns 64-127) in
+  // the second X buffer, which nothing writes before the first slab's barrier.
+  int *qbad = reinterpret_cast<int *>(&Xs[1][0]);
   const int tid = tid_x();
   const int lane = tid % 64;
   const int wave = tid / 64;
@@ -185,22 +239,21 @@ __global__ __launch_bounds__(256) void msq_mfma_kernel(
     return;
   }
 
-  const int sr = tid / 4, sc = (tid % 4) * 16;
-  const int8_t *xp = Xq + (int64_t)(row0 + sr) * Kp + sc;
-  const int8_t *yp = Yq + (int64_t)(col0 + sr) * Kp + sc;
-  const int64_t half = (int64_t)64 * Kp;
+  // slab ks of the tile's row panel: Xq + (tr * Kp / 64 + ks) * 8 KB
+  const int8_t *xp = Xq + (int64_t)tr * (Kp / QK) * QSLAB + tid * 16;
+  const int8_t *yp = Yq + (int64_t)tc * (Kp / QK) * QSLAB + tid * 16;
   uint4 x0, x1, y0, y1;
   auto load = [&]() {
     x0 = *reinterpret_cast<const uint4 *>(xp);
-    x1 = *reinterpret_cast<const uint4 *>(xp + half);
+    x1 = *reinterpret_cast<const uint4 *>(xp + QSLAB / 2);
     y0 = *reinterpret_cast<const uint4 *>(yp);
-    y1 = *reinterpret_cast<const uint4 *>(yp + half);
+    y1 = *reinterpret_cast<const uint4 *>(yp + QSLAB / 2);
   };
   auto stage = [&](int b) {
-    *reinterpret_cast<uint4 *>(&Xs[b][sr * QLD + sc]) = x0;
-    *reinterpret_cast<uint4 *>(&Xs[b][(sr + 64) * QLD + sc]) = x1;
-    *reinterpret_cast<uint4 *>(&Ys[b][sr * QLD + sc]) = y0;
-    *reinterpret_cast<uint4 *>(&Ys[b][(sr + 64) * QLD + sc]) = y1;
+    *reinterpret_cast<uint4 *>(&Xs[b][tid * 16]) = x0;
+    *reinterpret_cast<uint4 *>(&Xs[b][QSLAB / 2 + tid * 16]) = x1;
+    *reinterpret_cast<uint4 *>(&Ys[b][tid * 16]) = y0;
+    *reinterpret_cast<uint4 *>(&Ys[b][QSLAB / 2 + tid * 16]) = y1;
   };
 
   i32x4 acc[4][4];
@@ -213,22 +266,22 @@ __global__ __launch_bounds__(256) void msq_mfma_kernel(
   load();
   stage(0);
   __syncthreads();
-  const int fr = lane % 16, fk = (lane / 16) * 16;
+  const int fr = lane % 16;
   for (int ks = 0; ks < nks; ++ks) {
     const int buf = ks & 1;
     const bool more = ks + 1 < nks;
     if (more) {
-      xp += QK;
-      yp += QK;
+      xp += QSLAB;
+      yp += QSLAB;
       load();
     }
     i32x4 a[4], b[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      a[i] = *reinterpret_cast<const i32x4 *>(&Xs[buf][(wr * 64 + i * 16 + fr) * QLD + fk]);
+      a[i] = *reinterpret_cast<const i32x4 *>(&Xs[buf][(wr * 4 + i) * 1024 + lane * 16]);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      b[j] = *reinterpret_cast<const i32x4 *>(&Ys[buf][(wc * 64 + j * 16 + fr) * QLD + fk]);
+      b[j] = *reinterpret_cast<const i32x4 *>(&Ys[buf][(wc * 4 + j) * 1024 + lane * 16]);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -247,18 +300,20 @@ __global__ __launch_bounds__(256) void msq_mfma_kernel(
     const int c = qc0 + j * 16 + fr;
     nyv[j] = c < Cn ? ny[c] : 0;
   }
+  // For valid rows and columns the sum s = sum_k (x - y)^2 is below
+  // 65536 * 255^2 < 2^32, so it is exact in 32-bit unsigned arithmetic.
   bool bad = mybad;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = qr0 + i * 16 + (lane / 16) * 4 + v;
-      const int nxv = r < R ? nx[r] : 0;
+      const uint32_t nxv = r < R ? (uint32_t)nx[r] : 0u;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int c = qc0 + j * 16 + fr;
-        const int64_t s = (int64_t)nxv + nyv[j] - 2 * (int64_t)acc[i][j][v];
-        if (r < R && c < Cn && s > EXACT_LIMIT) bad = true;
+        const uint32_t s = nxv + (uint32_t)nyv[j] - 2u * (uint32_t)acc[i][j][v];
+        if (r < R && c < Cn && s > (uint32_t)EXACT_LIMIT) bad = true;
       }
     }
   if (__ballot(bad) != 0) {
@@ -276,8 +331,7 @@ __global__ __launch_bounds__(256) void msq_mfma_kernel(
       for (int j = 0; j < 4; ++j) {
         const int c = qc0 + j * 16 + fr;
         if (c >= Cn) continue;
-        // 0 <= s <= 2^24 here: the conversion is exact
-        float d = (float)(nxv + nyv[j] - 2 * acc[i][j][v]);
+        float d = (float)(nxv + nyv[j] - 2 * acc[i][j][v]);  // 0 <= s <= 2^24
         if (C) d = d + C[(int64_t)r * ldc + c];
         O[(int64_t)r * ldo + c] = d;
       }
