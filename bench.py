@@ -7,6 +7,12 @@ minus_square; or_and on bit-packed Bernoulli(0.02) matrices), inputs resident
 in HBM before timing.  `value` is the step's aggregate semiring throughput,
 sum over ops of 2*M*N*K / step time, in GOPS.
 
+minus_square on this integer data runs on the int8 matrix cores as
+|a|^2 + |b|^2 - 2 a.b, which is exact and bit-identical to the k-ordered FP32
+sum wherever the sums stay at or below 2^24 (all of them here); its per-op
+entry also gives the direct FP32 kernel's time and the fraction of the int8
+MFMA peak.
+
 Also reported: per-op GOPS and fraction of the FP32 vector peak, min_plus at
 8192^3, the n=1024 APSP converge loop (reference loop structure and fused
 check), the roofline of the dominant kernel (min_plus, timed with HIP events on
@@ -29,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_VECTOR_TFLOPS = 157.3   # MI355X published FP32 vector peak (FMA, packed)
+PEAK_INT8_MFMA_TOPS = 5000.0      # MI355X published dense int8 MFMA peak
 # VALU issue ceiling of the min_plus inner step (2 x v_add_f32 + 1 x v_min3_f32
 # per 4 semiring ops; packed FP32 is not used).  gfx950 dual-issues the adds but
 # not v_min3: a microbenchmark of that mix issues at most 1.60 wave-instructions
@@ -96,7 +103,19 @@ def make_call(torch, simd2_amd, op, n, seed):
         st = fn(*args)
         if st:
             raise RuntimeError(f"{op} status {st}")
-    return call, (A, B, C, D)
+    if op != "minus_square":
+        return call, (A, B, C, D)
+    # the direct FP32 kernel on every tile (an explicit tile configuration),
+    # into its own output
+    D2 = torch.empty_like(C)
+    args2 = (int(simd2_amd.OP[op]), n, n, n, B.data_ptr(), n, A.data_ptr(), n, C.data_ptr(),
+             n, D2.data_ptr(), n, 1, 1, stream)
+
+    def direct():
+        st = L.simd2_srgemm_ex(*args2)
+        if st:
+            raise RuntimeError(f"{op} direct status {st}")
+    return call, (A, B, C, D, D2), direct
 
 
 def time_launches(torch, call, reps):
@@ -396,6 +415,16 @@ def main():
         per_op[op] = {"ms": round(ms, 4), "GOPS": round(tops * 1e3, 1),
                       "frac_fp32_vector_peak": round(tops / PEAK_FP32_VECTOR_TFLOPS, 4)}
         log(f"{op}: {ms:.3f} ms  {tops:.2f} T ops/s")
+    # minus_square: which tiles took the int8 path, and the direct kernel beside it
+    mf, va = simd2_amd.minus_square_tiles()
+    ms_dir = time_launches(torch, calls["minus_square"][2], max(3, args.steps // 2))
+    msq = per_op["minus_square"]
+    msq.update({"path": "int8 MFMA, exact (|a|^2+|b|^2-2a.b, sums <= 2^24)",
+                "int8_tiles": mf, "direct_tiles": va,
+                "frac_int8_mfma_peak": round(msq["GOPS"] / 1e3 / PEAK_INT8_MFMA_TOPS, 4),
+                "direct_fp32_kernel_ms": round(ms_dir, 4),
+                "speedup_vs_direct": round(ms_dir / msq["ms"], 2)})
+    log(f"minus_square int8 tiles {mf}, direct tiles {va}, direct kernel {ms_dir:.3f} ms")
 
     mp = per_op["min_plus"]
     mp_tops = mp["GOPS"] / 1e3
