@@ -16,8 +16,10 @@ MFMA peak.
 Also reported: per-op GOPS and fraction of the FP32 vector peak, min_plus at
 8192^3, the n=1024 APSP converge loop (reference loop structure and fused
 check), the roofline of the dominant kernel (min_plus, timed with HIP events on
-the stream it is launched on), and the CPU oracle timed on this host on a
-row-sampled slice of the same min_plus workload.
+the stream it is launched on; per-op times are the median of 20 back-to-back
+launches), and the CPU oracle timed on this host on a row-sampled slice of the
+same min_plus workload, plus its full n=1024 APSP loop on all host cores,
+checked bit for bit against the GPU's.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 3]
 """
@@ -118,6 +120,29 @@ def make_call(torch, simd2_amd, op, n, seed):
     return call, (A, B, C, D, D2), direct
 
 
+def time_launches_median(torch, call, reps=20):
+    """Median device time of `reps` back-to-back launches, each bracketed by
+    HIP events on the launching stream (torch's current stream)."""
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record()
+    for i in range(reps):
+        call()
+        ev[i + 1].record()
+    ev[-1].synchronize()
+    return float(np.median([ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]))
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def time_launches(torch, call, reps):
     """Average device time per launch, HIP events on the launching stream."""
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -129,8 +154,11 @@ def time_launches(torch, call, reps):
     return s.elapsed_time(e) / reps
 
 
-def cpu_baseline(n=4096):
-    """The CPU oracle on this host: min_plus rows of the 4096^3 workload."""
+def cpu_baseline(n=4096, apsp=None):
+    """The CPU oracle on this host: min_plus rows of the 4096^3 workload, and
+    (when `apsp` = (X0, X_gpu, squarings) of the n=1024 APSP run is given)
+    the full converge loop of that graph on all host cores, checked against
+    the GPU's matrix bit for bit."""
     import oracle
     oracle.build()
     rng = np.random.default_rng(0)
@@ -166,11 +194,23 @@ def cpu_baseline(n=4096):
     out["openmp"] = {"value": 2.0 * rows * n * n / t2 / 1e9, "unit": "GOPS", "cores": cores,
                      "kind": "port", "sample": f"min_plus rows 0..{rows - 1} of 4096^3 "
                      f"(row-sampled, OpenMP over rows, bit-identical, {t2:.1f} s)"}
+    out["host_cpu"] = host_cpu_model()
+    if apsp is not None:
+        X0, Xg, itr_g = apsp
+        t0 = time.perf_counter()
+        Xc, itr_c = oracle.converge_f32("min_plus", X0, epilogue=True, check="threshold",
+                                        threads=cores)
+        t3 = time.perf_counter() - t0
+        out["apsp_er1024_loop"] = {
+            "ms": round(t3 * 1e3, 1), "cores": cores, "kind": "port", "squarings": itr_c,
+            "identical_to_gpu": bool(itr_c == itr_g and np.array_equal(
+                Xc.view(np.uint32), np.ascontiguousarray(Xg).view(np.uint32)))}
     return out
 
 
-def apsp_1024(torch, simd2_amd):
-    """n=1024 ER p=0.05 seed 0 APSP: converge-loop wall time per solve."""
+def apsp_1024(torch, simd2_amd, keep=None):
+    """n=1024 ER p=0.05 seed 0 APSP: converge-loop wall time per solve.
+    `keep` (a list) receives (X0, final matrix, squarings) on the host."""
     from simd2_amd.graphs import er_graph
     X0 = er_graph(torch, 1024, 0.05, seed=0)
     res = {}
@@ -185,6 +225,8 @@ def apsp_1024(torch, simd2_amd):
                                             flags=flags)
             times.append((time.perf_counter() - t0) * 1e3)
         res[name] = {"ms": float(np.median(times[1:])), "squarings": itr}
+    if keep is not None:
+        keep.append((X0.cpu().numpy(), X.cpu().numpy(), itr))
     return res
 
 
@@ -410,7 +452,7 @@ def main():
 
     per_op = {}
     for op in OPS:
-        ms = time_launches(torch, calls[op][0], max(5, args.steps))
+        ms = time_launches_median(torch, calls[op][0], 20)
         tops = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
         per_op[op] = {"ms": round(ms, 4), "GOPS": round(tops * 1e3, 1),
                       "frac_fp32_vector_peak": round(tops / PEAK_FP32_VECTOR_TFLOPS, 4)}
@@ -444,6 +486,7 @@ def main():
     torch.cuda.empty_cache()
 
     extras = {}
+    apsp_keep = []
     if not args.no_extras:
         call, keep = make_call(torch, simd2_amd, "min_plus", 8192, seed=77)
         call()
@@ -456,7 +499,7 @@ def main():
         log(f"min_plus 8192^3: {ms:.3f} ms {tops:.2f} T ops/s")
         del keep, call
         torch.cuda.empty_cache()
-        extras["apsp_er1024"] = apsp_1024(torch, simd2_amd)
+        extras["apsp_er1024"] = apsp_1024(torch, simd2_amd, apsp_keep)
         log(f"apsp n=1024: {extras['apsp_er1024']}")
         extras["pld_knn"] = pld_knn(torch, simd2_amd)
         log(f"pld knn: {extras['pld_knn']}")
@@ -470,9 +513,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        base = cpu_baseline()
+        base = cpu_baseline(apsp=apsp_keep[0] if apsp_keep else None)
         cpu = dict(base["single"])
         cpu["openmp"] = base["openmp"]
+        cpu["host_cpu"] = base["host_cpu"]
+        if "apsp_er1024_loop" in base:
+            cpu["apsp_er1024_loop"] = base["apsp_er1024_loop"]
         log(f"cpu baseline: {cpu}")
 
     line = {
