@@ -169,10 +169,11 @@ int simd2_orand_packed(int M, int N, int K, const uint32_t *A, int lda_words,
                        int ldc_words, uint32_t *D, int ldd_words,
                        int do_epilogue, hipStream_t stream);
 
-/* 64x64 tiles of the last minus_square call on `stream` (current device)
- * computed on the int8 matrix cores and on the direct FP32 kernel (no
- * counterpart in the reference; for tests and reports).  Synchronises
- * `stream`. */
+/* 64x64 tiles of the last minus_square call on `stream` (current device;
+ * calls without an explicit tile configuration) computed on the int8 matrix
+ * cores and on the direct FP32 kernel; both 0 before the first such call
+ * that reached the int8 path on that stream (no counterpart in the
+ * reference; for tests and reports).  Synchronises `stream`. */
 int simd2_minus_square_tiles(hipStream_t stream, int *mfma_tiles, int *valu_tiles);
 
 /* Frees the scratch buffers the minus_square int8 path keeps per (device,

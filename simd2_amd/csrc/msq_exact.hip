@@ -383,6 +383,20 @@ Scratch *scratch_for(int dev, hipStream_t s, size_t bytes) {
   return &g_scratch.back();
 }
 
+// A call that takes the direct kernel on every tile: the statistics of this
+// (device, stream) then report all of its tiles as direct.
+void record_direct_only(hipStream_t s, int R, int Cn) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  for (auto &e : g_scratch)
+    if (e.dev == dev && e.s == s) {
+      e.gate = nullptr;
+      e.tiles_r = R > 0 ? (R + 63) / 64 : 0;
+      e.tiles_c = Cn > 0 ? (Cn + 63) / 64 : 0;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_minus_square(int R, int Cn, int K, const float *X, int64_t ldx,
@@ -393,13 +407,16 @@ hipError_t launch_minus_square(int R, int Cn, int K, const float *X, int64_t ldx
   ep.ldc = ldc;
   // K <= 65536 keeps every norm below 2^30 and every int32 dot product in
   // range.
-  if (R <= 0 || Cn <= 0 || K <= 0 || K > 65536)
+  if (R <= 0 || Cn <= 0 || K <= 0 || K > 65536) {
+    record_direct_only(s, R, Cn);
     return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
+  }
   // Under graph capture the scratch buffer could be freed while a graph
   // still refers to it: take the direct path there.
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
     (void)hipGetLastError();
+    record_direct_only(s, R, Cn);
     return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
   }
   int dev = 0;
@@ -414,7 +431,15 @@ hipError_t launch_minus_square(int R, int Cn, int K, const float *X, int64_t ldx
 
   std::lock_guard<std::mutex> lock(g_scratch_mu);
   Scratch *sc = scratch_for(dev, s, bytes);
-  if (!sc) return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
+  if (!sc) {
+    for (auto &e2 : g_scratch)  // lock held: record_direct_only would deadlock
+      if (e2.dev == dev && e2.s == s) {
+        e2.gate = nullptr;
+        e2.tiles_r = (R + 63) / 64;
+        e2.tiles_c = (Cn + 63) / 64;
+      }
+    return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
+  }
   char *base = static_cast<char *>(sc->p);
   int8_t *Xq = reinterpret_cast<int8_t *>(base);
   int8_t *Yq = reinterpret_cast<int8_t *>(base + bx);
@@ -449,7 +474,10 @@ hipError_t minus_square_tile_counts(hipStream_t s, int *mfma_tiles, int *valu_ti
   if (e != hipSuccess) return e;
   std::lock_guard<std::mutex> lock(g_scratch_mu);
   for (auto &sc : g_scratch)
-    if (sc.dev == dev && sc.s == s && sc.gate) {
+    if (sc.dev == dev && sc.s == s && !sc.gate) {
+      *valu_tiles = sc.tiles_r * sc.tiles_c;
+      return hipSuccess;
+    } else if (sc.dev == dev && sc.s == s) {
       std::vector<int> g((size_t)sc.tiles_r * sc.gate_ld);
       e = hipStreamSynchronize(s);
       if (e == hipSuccess)

@@ -159,6 +159,23 @@ def test_large_square_matches_direct_kernel(gpu):
     assert (mf, va) == (tiles(n, n), 0)
 
 
+def test_k_past_the_int8_limit_takes_the_direct_kernel(gpu, oracle_mod):
+    """K = 70000 (past the 65536 the int8 path accepts) after an int8-path
+    call on the same stream: the result is the oracle's and every tile is
+    reported as direct."""
+    rng = np.random.default_rng(23)
+    A, B, C = ints(rng, (64, 64), 1, 3), ints(rng, (64, 64), 1, 3), ints(rng, (64, 64), 1, 3)
+    D, (mf, va) = run(gpu, A, B, C, True)
+    assert (mf, va) == (1, 0)
+    M, N, K = 3, 70, 70000
+    A, B, C = ints(rng, (M, K), 0, 1), ints(rng, (K, N), 0, 1), ints(rng, (M, N), 0, 1)
+    say("K = 70000, direct kernel")
+    D, (mf, va) = run(gpu, A, B, C, True)
+    want = oracle_mod.srgemm_f32("minus_square", A, B, C, True, threads=8)
+    assert np.array_equal(bits(D), bits(want))
+    assert (mf, va) == (0, tiles(M, N)), (mf, va)
+
+
 def test_scratch_growth_streams_and_release(gpu, oracle_mod):
     """Small, larger, small again on one stream (the buffer grows once), the
     same on a second stream, then the buffers freed and used again."""
