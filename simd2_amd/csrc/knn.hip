@@ -150,6 +150,17 @@ __global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Minimum over the 16 lanes of a DPP row (lanes 16j .. 16j+15), in every lane
+// of the row: rotations by 8, 4, 2 and 1 within the row, each one VALU min
+// with a DPP operand (no LDS traffic).  The whole wave must be active.
+__device__ __forceinline__ float row_min16(float m) {
+  m = fminf(m, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(m), 0x128, 0xf, 0xf, false)));
+  m = fminf(m, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(m), 0x124, 0xf, 0xf, false)));
+  m = fminf(m, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(m), 0x122, 0xf, 0xf, false)));
+  m = fminf(m, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(m), 0x121, 0xf, 0xf, false)));
+  return m;
+}
+
 // One wave per point: the FP32 coordinates split into a BF16 high part and a
 // BF16 low part (x - hi), dims zero-padded to dpad, stored [point][hi | lo],
 // and the FP32 squared norm (wave-summed).
@@ -319,9 +330,9 @@ __global__ __launch_bounds__(256) void knn_pass_kernel(
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb)
             if (valid[rb]) m = fminf(m, qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v]);
-          // the 16 lanes of this query (same lane / 16) hold the tile's 64 references
-#pragma unroll
-          for (int off = 1; off < 16; off <<= 1) m = fminf(m, __shfl_xor(m, off));
+          // the 16 lanes of this query (same lane / 16, one DPP row) hold the
+          // tile's 64 references
+          m = row_min16(m);
           if (lane % 16 == 0 && qv[g][v] < Q) tmin[(int64_t)qv[g][v] * ntiles + r0 / CR] = m;
         }
       }
