@@ -412,12 +412,20 @@ __global__ __launch_bounds__(256) void knn_tau_kernel(const float *__restrict__ 
   } else {
     while (lo < hi) {
       const uint32_t mid = lo + (hi - lo) / 2;
+      // The wave's count of keys <= mid: one ballot per register of keys,
+      // counted in scalar registers (no cross-lane shuffles); keys past the
+      // registers (ntiles > 512) are counted per lane and summed.
       int c = 0;
 #pragma unroll
-      for (int i = 0; i < TP; ++i) c += (lane + 64 * i < ntiles && kv[i] <= mid) ? 1 : 0;
-      for (int j = lane + 64 * TP; j < ntiles; j += 64) c += fkey(t[j]) <= mid ? 1 : 0;
+      for (int i = 0; i < TP; ++i)
+        c += (int)__popcll(__ballot(lane + 64 * i < ntiles && kv[i] <= mid));
+      if (ntiles > 64 * TP) {
+        int e = 0;
+        for (int j = lane + 64 * TP; j < ntiles; j += 64) e += fkey(t[j]) <= mid ? 1 : 0;
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
+        c += e;
+      }
       if (c >= CAND_K)
         hi = mid;
       else
