@@ -343,6 +343,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void m
 // demand.  The whole launch sequence runs under the lock, so a buffer is
 // never freed between the time a call takes it and the time its kernels are
 // enqueued; growth synchronises the stream before freeing the old buffer.
+// Entries are keyed by the stream handle: a stream destroyed with this work
+// still pending and a new stream given the same handle would share one.
 struct Scratch {
   int dev;
   hipStream_t s;
@@ -384,17 +386,21 @@ Scratch *scratch_for(int dev, hipStream_t s, size_t bytes) {
 }
 
 // A call that takes the direct kernel on every tile: the statistics of this
-// (device, stream) then report all of its tiles as direct.
-void record_direct_only(hipStream_t s, int R, int Cn) {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return;
-  std::lock_guard<std::mutex> lock(g_scratch_mu);
+// (device, stream) then report all of its tiles as direct.  The _locked form
+// expects g_scratch_mu held.
+void record_direct_only_locked(int dev, hipStream_t s, int R, int Cn) {
   for (auto &e : g_scratch)
     if (e.dev == dev && e.s == s) {
       e.gate = nullptr;
       e.tiles_r = R > 0 ? (R + 63) / 64 : 0;
       e.tiles_c = Cn > 0 ? (Cn + 63) / 64 : 0;
     }
+}
+void record_direct_only(hipStream_t s, int R, int Cn) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  record_direct_only_locked(dev, s, R, Cn);
 }
 
 }  // namespace
@@ -432,12 +438,7 @@ hipError_t launch_minus_square(int R, int Cn, int K, const float *X, int64_t ldx
   std::lock_guard<std::mutex> lock(g_scratch_mu);
   Scratch *sc = scratch_for(dev, s, bytes);
   if (!sc) {
-    for (auto &e2 : g_scratch)  // lock held: record_direct_only would deadlock
-      if (e2.dev == dev && e2.s == s) {
-        e2.gate = nullptr;
-        e2.tiles_r = (R + 63) / 64;
-        e2.tiles_c = (Cn + 63) / 64;
-      }
+    record_direct_only_locked(dev, s, R, Cn);
     return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
   }
   char *base = static_cast<char *>(sc->p);
