@@ -43,6 +43,15 @@ PEAK_INT8_MFMA_TOPS = 5000.0      # MI355X published dense int8 MFMA peak
 # not v_min3: a microbenchmark of that mix issues at most 1.60 wave-instructions
 # per CU per cycle, so the ceiling is 1.60 x 64 lanes x 4/3 ops x 256 CUs x 2.4 GHz.
 ISSUE_CEILING_TOPS = 1.60 * 64 * 4.0 / 3.0 * 256 * 2.4e9 / 1e12
+# The same measured rates for the other VALU mixes: min/max-only (2 x v_min +
+# v_max3 per 4 ops) issues at most 0.90 wave-instructions per CU per cycle;
+# the direct minus_square (v_sub + v_fmac per term, 2 ops) at 1.60.
+ISSUE_CEILING_BY_OP = {
+    "min_plus": ISSUE_CEILING_TOPS, "max_plus": ISSUE_CEILING_TOPS,
+    "min_mul": ISSUE_CEILING_TOPS, "max_mul": ISSUE_CEILING_TOPS,
+    "max_min": 0.90 * 64 * 4.0 / 3.0 * 256 * 2.4e9 / 1e12,
+    "min_max": 0.90 * 64 * 4.0 / 3.0 * 256 * 2.4e9 / 1e12,
+}
 # Bytes moved past L2 per launch of the min_plus kernel at n = 4096 (128x64
 # tiles): rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, the factors a 48 MiB
 # streaming copy reads on the same counters (profiles/srgemm_4096_counters.md):
@@ -479,9 +488,19 @@ def main():
                 "frac_issue_ceiling": round(mp_tops / ISSUE_CEILING_TOPS, 4),
                 "traffic": TRAFFIC.get(n),
                 "note": "achieved = 2*n^3 semiring ops (one (x) and one (+) per i,j,k) / "
-                        "mean launch time on the launching stream, in T semiring ops/s; "
+                        "median launch time on the launching stream, in T semiring ops/s; "
                         "issue_ceiling = measured VALU issue rate of the kernel's "
                         "instruction mix (2 dual-issuable + 1 single-issue per 4 ops)"}
+    # the longest kernel of the step (min_plus is the one the target names)
+    lop = max(per_op, key=lambda o: per_op[o]["ms"])
+    ltops = per_op[lop]["GOPS"] / 1e3
+    roofline["longest_kernel"] = {
+        "op": lop, "ms": per_op[lop]["ms"], "achieved": round(ltops, 3),
+        "frac": round(ltops / PEAK_FP32_VECTOR_TFLOPS, 4),
+        "issue_ceiling": round(ISSUE_CEILING_BY_OP[lop], 2) if lop in ISSUE_CEILING_BY_OP
+        else None,
+        "frac_issue_ceiling": round(ltops / ISSUE_CEILING_BY_OP[lop], 4)
+        if lop in ISSUE_CEILING_BY_OP else None}
     calls.clear()
     torch.cuda.empty_cache()
 
