@@ -7,11 +7,11 @@ minus_square; or_and on bit-packed Bernoulli(0.02) matrices), inputs resident
 in HBM before timing.  `value` is the step's aggregate semiring throughput,
 sum over ops of 2*M*N*K / step time, in GOPS.
 
-minus_square on this integer data runs on the int8 matrix cores as
-|a|^2 + |b|^2 - 2 a.b, which is exact and bit-identical to the k-ordered FP32
-sum wherever the sums stay at or below 2^24 (all of them here); its per-op
-entry also gives the direct FP32 kernel's time and the fraction of the int8
-MFMA peak.
+minus_square runs through simd2_minus_square_exact (workspace allocated before
+timing): on this integer data the int8 matrix cores compute |a|^2 + |b|^2 -
+2 a.b, which is exact and bit-identical to the k-ordered FP32 sum wherever the
+sums stay at or below 2^24 (all of them here); its per-op entry also gives the
+drop-in's (direct FP32 kernel) time and the fraction of the int8 MFMA peak.
 
 Also reported: per-op GOPS and fraction of the FP32 vector peak, min_plus at
 8192^3, the n=1024 APSP converge loop (reference loop structure and fused
@@ -116,17 +116,27 @@ def make_call(torch, simd2_amd, op, n, seed):
             raise RuntimeError(f"{op} status {st}")
     if op != "minus_square":
         return call, (A, B, C, D)
-    # the direct FP32 kernel on every tile (an explicit tile configuration),
-    # into its own output
+    # minus_square: the exact int8 entry point, its workspace allocated here,
+    # outside any timed region; the drop-in (the direct FP32 kernel) into its
+    # own output beside it
+    ws = torch.empty(simd2_amd.minus_square_workspace(n, n, n), dtype=torch.uint8,
+                     device="cuda")
     D2 = torch.empty_like(C)
-    args2 = (int(simd2_amd.OP[op]), n, n, n, B.data_ptr(), n, A.data_ptr(), n, C.data_ptr(),
-             n, D2.data_ptr(), n, 1, 1, stream)
+    args_x = (n, n, n, B.data_ptr(), n, A.data_ptr(), n, C.data_ptr(), n, D.data_ptr(), n, 1,
+              ws.data_ptr(), ws.numel(), stream)
+    args_d = (n, n, n, B.data_ptr(), n, A.data_ptr(), n, C.data_ptr(), n, D2.data_ptr(), True,
+              stream)
+
+    def exact():
+        st = L.simd2_minus_square_exact(*args_x)
+        if st:
+            raise RuntimeError(f"{op} exact status {st}")
 
     def direct():
-        st = L.simd2_srgemm_ex(*args2)
+        st = fn(*args_d)
         if st:
             raise RuntimeError(f"{op} direct status {st}")
-    return call, (A, B, C, D, D2), direct
+    return exact, (A, B, C, D, D2, ws), direct
 
 
 def time_launches_median(torch, call, reps=20):
@@ -467,13 +477,14 @@ def main():
                       "frac_fp32_vector_peak": round(tops / PEAK_FP32_VECTOR_TFLOPS, 4)}
         log(f"{op}: {ms:.3f} ms  {tops:.2f} T ops/s")
     # minus_square: which tiles took the int8 path, and the direct kernel beside it
-    mf, va = simd2_amd.minus_square_tiles()
+    mf, va = simd2_amd.minus_square_tiles(calls["minus_square"][1][-1], n, n, n)
     ms_dir = time_launches(torch, calls["minus_square"][2], max(3, args.steps // 2))
     msq = per_op["minus_square"]
-    msq.update({"path": "int8 MFMA, exact (|a|^2+|b|^2-2a.b, sums <= 2^24)",
+    msq.update({"path": "simd2_minus_square_exact: int8 MFMA, exact (|a|^2+|b|^2-2a.b, "
+                        "sums <= 2^24), caller's workspace",
                 "int8_tiles": mf, "direct_tiles": va,
                 "frac_int8_mfma_peak": round(msq["GOPS"] / 1e3 / PEAK_INT8_MFMA_TOPS, 4),
-                "direct_fp32_kernel_ms": round(ms_dir, 4),
+                "dropin_direct_fp32_ms": round(ms_dir, 4),
                 "speedup_vs_direct": round(ms_dir / msq["ms"], 2)})
     log(f"minus_square int8 tiles {mf}, direct tiles {va}, direct kernel {ms_dir:.3f} ms")
 

@@ -24,6 +24,7 @@
 #define SIMD2_SRGEMM_H
 
 #include <stdbool.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <hip/hip_runtime_api.h>
 
@@ -109,14 +110,9 @@ int cuasr_orand_srsgemm(int M, int N, int K, int const *A, int lda,
                         int const *B, int ldb, int *C, int ldc, int *D,
                         bool do_epilogue_min, hipStream_t stream SIMD2_DEFAULT_STREAM);
 /* replaces cuasr_plusmiussquare_srsgemm (the reference's spelling),
- * kernel/srgemm.cuh:552-626: D = [C +] sum_k (A(i,k) - B(k,j))^2.
- * Each 64x64 tile of D whose rows of A and columns of B hold only integers
- * in [-128, 127] and whose sums are at most 2^24 is computed on the int8
- * matrix cores as |a|^2 + |b|^2 - 2 a.b, exactly; for such a tile the FP32
- * sum k by k of host_srgemm is that same integer, so D is bit-identical.
- * Other tiles run the direct FP32 kernel.  The int8 path keeps one scratch
- * buffer per (device, stream), about (M + N) * K bytes, until
- * simd2_release_scratch; it is not used while the stream is being captured. */
+ * kernel/srgemm.cuh:552-626: D = [C +] sum_k (A(i,k) - B(k,j))^2, the direct
+ * FP32 kernel.  Like every drop-in it allocates nothing; the exact int8 form
+ * with a caller-owned workspace is simd2_minus_square_exact below. */
 int cuasr_plusmiussquare_srsgemm(int M, int N, int K, float const *A, int lda,
                                  float const *B, int ldb, float *C, int ldc,
                                  float *D, bool do_epilogue_min,
@@ -139,8 +135,7 @@ int simd2_srgemm(int op, int M, int N, int K, const void *A, int lda,
  * time, kernel/srgemm.cuh:574-583): -1 = automatic, 0 = 128x128 tiles (8x8
  * per thread), 1 = 64x64 (4x4), 2 = 64x128 (4x8 per thread; 64 along M and
  * 128 along N of the column-major call).  Used by the autotuner and benchmarks; results are
- * identical for every configuration.  For minus_square an explicit
- * configuration runs the direct FP32 kernel on every tile (no int8 path). */
+ * identical for every configuration. */
 int simd2_srgemm_ex(int op, int M, int N, int K, const void *A, int lda, const void *B,
                     int ldb, const void *C, int ldc, void *D, int ldd, int do_epilogue,
                     int tile_config, hipStream_t stream);
@@ -169,17 +164,27 @@ int simd2_orand_packed(int M, int N, int K, const uint32_t *A, int lda_words,
                        int ldc_words, uint32_t *D, int ldd_words,
                        int do_epilogue, hipStream_t stream);
 
-/* 64x64 tiles of the last minus_square call on `stream` (current device;
- * calls without an explicit tile configuration) computed on the int8 matrix
- * cores and on the direct FP32 kernel; both 0 before the first such call
- * that reached the int8 path on that stream (no counterpart in the
- * reference; for tests and reports).  Synchronises `stream`. */
-int simd2_minus_square_tiles(hipStream_t stream, int *mfma_tiles, int *valu_tiles);
+/* minus_square on the int8 matrix cores, exactly (no counterpart in the
+ * reference; column-major like simd2_srgemm).  Each 64x64 tile of D whose
+ * rows of A and columns of B hold only integers in [-128, 127] and whose
+ * sums are at most 2^24 is computed as |a|^2 + |b|^2 - 2 a.b in integer
+ * arithmetic: for such a tile the FP32 sum k by k of host_srgemm is that
+ * same integer, so D is bit-identical to cuasr_plusmiussquare_srsgemm's.
+ * Other tiles run the direct FP32 kernel.  `workspace` (device memory, kept
+ * by the caller) must hold simd2_minus_square_workspace(M, N, K) bytes, about
+ * (M + N) * K; that size is 0 when K > 65536, and then every tile runs the
+ * direct kernel.  Calls sharing a workspace must be ordered (one stream). */
+size_t simd2_minus_square_workspace(int M, int N, int K);
+int simd2_minus_square_exact(int M, int N, int K, const float *A, int lda, const float *B,
+                             int ldb, const float *C, int ldc, float *D, int ldd,
+                             int do_epilogue, void *workspace, size_t workspace_bytes,
+                             hipStream_t stream);
 
-/* Frees the scratch buffers the minus_square int8 path keeps per (device,
- * stream), after synchronising each device that holds one (no counterpart in
- * the reference). */
-int simd2_release_scratch(void);
+/* 64x64 tiles of the last simd2_minus_square_exact call on this workspace
+ * and problem that ran on the int8 matrix cores and on the direct kernel
+ * (for tests and reports).  Synchronises `stream`. */
+int simd2_minus_square_tiles(const void *workspace, int M, int N, int K, hipStream_t stream,
+                             int *mfma_tiles, int *valu_tiles);
 
 /* Library / device information for reporting (clock, CU count). */
 int simd2_device_info(int *cu_count, int *clock_khz, char *name, int name_len);

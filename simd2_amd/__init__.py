@@ -120,10 +120,14 @@ def lib() -> ctypes.CDLL:
         L.simd2_status_string.restype = ctypes.c_char_p
         L.simd2_device_info.argtypes = [ctypes.POINTER(i), ctypes.POINTER(i), ctypes.c_char_p, i]
         L.simd2_device_info.restype = i
-        L.simd2_minus_square_tiles.argtypes = [p, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.simd2_minus_square_workspace.argtypes = [i, i, i]
+        L.simd2_minus_square_workspace.restype = ctypes.c_size_t
+        L.simd2_minus_square_exact.argtypes = [i, i, i, p, i, p, i, p, i, p, i, i, p,
+                                               ctypes.c_size_t, p]
+        L.simd2_minus_square_exact.restype = i
+        L.simd2_minus_square_tiles.argtypes = [p, i, i, i, p, ctypes.POINTER(i),
+                                               ctypes.POINTER(i)]
         L.simd2_minus_square_tiles.restype = i
-        L.simd2_release_scratch.argtypes = []
-        L.simd2_release_scratch.restype = i
         for name in ("comp_update", "comp_update2", "comp_update_int"):
             fn = getattr(L, name)
             fn.argtypes = [p, p, p, p, i, i]
@@ -344,18 +348,43 @@ def h2f(x):
     return out
 
 
-def minus_square_tiles(stream=None):
-    """(64x64 tiles on the int8 matrix cores, tiles on the direct FP32 kernel)
-    of the last minus_square call on ``stream`` (simd2_minus_square_tiles)."""
+def minus_square_workspace(M, N, K):
+    """Workspace bytes of minus_square_exact for row-major A (M x K), B (K x N)."""
+    return lib().simd2_minus_square_workspace(N, M, K)
+
+
+def minus_square_exact(A, B, C=None, epilogue=False, out=None, workspace=None, stream=None):
+    """Row-major ``D = [C +] sum_k (A[i,k] - B[k,j])^2`` through
+    simd2_minus_square_exact: tiles of small-integer operands on the int8
+    matrix cores (bit-identical to the direct kernel), the rest direct.
+    ``workspace`` (a uint8 device tensor) is allocated when not given.
+    Returns (D, workspace)."""
+    torch = _torch()
+    M, K = A.shape
+    K2, N = B.shape
+    if K != K2:
+        raise ValueError("inner dimensions differ")
+    if out is None:
+        out = torch.empty((M, N), dtype=torch.float32, device=A.device)
+    need = minus_square_workspace(M, N, K)
+    if workspace is None:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=A.device)
+    st = lib().simd2_minus_square_exact(N, M, K, _ptr(B), max(N, 1), _ptr(A), max(K, 1),
+                                        _ptr(C) if epilogue else None, max(N, 1), _ptr(out),
+                                        max(N, 1), int(bool(epilogue)), _ptr(workspace),
+                                        workspace.numel(), _stream(stream))
+    _check(st, "simd2_minus_square_exact")
+    return out, workspace
+
+
+def minus_square_tiles(workspace, M, N, K, stream=None):
+    """(64x64 tiles on the int8 matrix cores, tiles on the direct kernel) of
+    the last minus_square_exact on ``workspace`` for this row-major problem."""
     mf, va = ctypes.c_int(0), ctypes.c_int(0)
-    _check(lib().simd2_minus_square_tiles(_stream(stream), ctypes.byref(mf), ctypes.byref(va)),
+    _check(lib().simd2_minus_square_tiles(_ptr(workspace), N, M, K, _stream(stream),
+                                          ctypes.byref(mf), ctypes.byref(va)),
            "simd2_minus_square_tiles")
     return mf.value, va.value
-
-
-def release_scratch():
-    """Free the minus_square int8 path's scratch buffers (simd2_release_scratch)."""
-    _check(lib().simd2_release_scratch(), "simd2_release_scratch")
 
 
 def device_info():

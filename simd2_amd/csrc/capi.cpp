@@ -64,8 +64,6 @@ int srgemm_f32_cm(int op, int M, int N, int K, const float *A, int lda, const fl
   hipError_t e;
   if (op == SIMD2_PLUS_MUL) {
     e = launch_plusmul_mfma(N, M, K, B, ldb, A, lda, D, ldd, epi ? C : nullptr, ldc, s);
-  } else if (op == SIMD2_MINUS_SQUARE && cfg == CFG_AUTO) {
-    e = launch_minus_square(N, M, K, B, ldb, A, lda, D, ldd, epi ? C : nullptr, ldc, s);
   } else {
     Epilogue ep;
     if (epi) {
@@ -220,12 +218,33 @@ int simd2_orand_packed(int M, int N, int K, const uint32_t *A, int lda_words,
                                         nullptr, 0, nullptr, stream));
 }
 
-int simd2_minus_square_tiles(hipStream_t stream, int *mfma_tiles, int *valu_tiles) {
-  if (!mfma_tiles || !valu_tiles) return SIMD2_ERROR_NULL_POINTER;
-  return hip_status(minus_square_tile_counts(stream, mfma_tiles, valu_tiles));
+size_t simd2_minus_square_workspace(int M, int N, int K) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  return minus_square_exact_workspace(N, M, K);  // row-major N x M problem
 }
 
-int simd2_release_scratch(void) { return hip_status(minus_square_release_scratch()); }
+int simd2_minus_square_exact(int M, int N, int K, const float *A, int lda, const float *B,
+                             int ldb, const float *C, int ldc, float *D, int ldd,
+                             int do_epilogue, void *workspace, size_t workspace_bytes,
+                             hipStream_t stream) {
+  const bool epi = do_epilogue != 0;
+  const int st = check_cm(M, N, K, A, lda, B, ldb, C, ldc, D, ldd, epi, sizeof(float));
+  if (st == OK_NOTHING_TO_DO) return SIMD2_SUCCESS;
+  if (st != SIMD2_SUCCESS) return st;
+  const size_t need = simd2_minus_square_workspace(M, N, K);
+  if (need > 0 && (!workspace || workspace_bytes < need)) return SIMD2_ERROR_WORKSPACE;
+  return hip_status(launch_minus_square_exact(N, M, K, B, ldb, A, lda, D, ldd,
+                                              epi ? C : nullptr, ldc, workspace,
+                                              workspace_bytes, stream));
+}
+
+int simd2_minus_square_tiles(const void *workspace, int M, int N, int K, hipStream_t stream,
+                             int *mfma_tiles, int *valu_tiles) {
+  if (!mfma_tiles || !valu_tiles) return SIMD2_ERROR_NULL_POINTER;
+  if (simd2_minus_square_workspace(M, N, K) > 0 && !workspace) return SIMD2_ERROR_WORKSPACE;
+  return hip_status(
+      minus_square_exact_tile_counts(workspace, N, M, K, stream, mfma_tiles, valu_tiles));
+}
 
 int simd2_device_info(int *cu_count, int *clock_khz, char *name, int name_len) {
   int dev = 0;

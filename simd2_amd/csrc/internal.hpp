@@ -45,14 +45,17 @@ hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,
 // minus_square, same row-major problem, with C added when set: tiles whose
 // operands are integers in [-128, 127] and whose sums stay <= 2^24 on the
 // int8 matrix cores (exact), the rest on the VALU kernel (msq_exact.hip).
-hipError_t launch_minus_square(int R, int Cn, int K, const float *X, int64_t ldx,
-                               const float *Y, int64_t ldy, float *O, int64_t ldo,
-                               const float *C, int64_t ldc, hipStream_t s);
-// 64x64 tiles of the last launch_minus_square on stream s (of the current
-// device) computed on the matrix cores / on the VALU kernel; synchronises s.
-hipError_t minus_square_tile_counts(hipStream_t s, int *mfma_tiles, int *valu_tiles);
-// Frees the scratch buffers of launch_minus_square (every device).
-hipError_t minus_square_release_scratch();
+// The workspace is the caller's; 0 bytes (K > 65536 or an empty problem)
+// means the VALU kernel does every tile.
+size_t minus_square_exact_workspace(int R, int Cn, int K);
+hipError_t launch_minus_square_exact(int R, int Cn, int K, const float *X, int64_t ldx,
+                                     const float *Y, int64_t ldy, float *O, int64_t ldo,
+                                     const float *C, int64_t ldc, void *ws, size_t ws_bytes,
+                                     hipStream_t s);
+// 64x64 tiles of the last launch_minus_square_exact on this workspace and
+// problem computed on the matrix cores / on the VALU kernel; synchronises s.
+hipError_t minus_square_exact_tile_counts(const void *ws, int R, int Cn, int K, hipStream_t s,
+                                          int *mfma_tiles, int *valu_tiles);
 
 // plus_mul on MFMA (v_mfma_f32_16x16x4_f32), same row-major problem.
 hipError_t launch_plusmul_mfma(int R, int Cn, int K, const float *X,

@@ -14,7 +14,8 @@
 // and columns passed the integer check and all of its sums are <= 2^24;
 // otherwise its gate entry asks the VALU kernel to compute it the direct way.
 //
-// Steps (one stream, no host synchronisation):
+// The caller supplies the workspace (minus_square_exact_workspace bytes);
+// nothing here allocates.  Steps (one stream, no host synchronisation):
 //   1. quantize: X rows and Y columns to int8 in fragment order (below),
 //      zero-padded to 128 rows / columns and to a multiple of 64 in k, with
 //      each row's / column's exact sum of squares (bit 31 set: some entry was
@@ -25,7 +26,6 @@
 //      (srgemm_valu.hip; every other workgroup returns at once).
 #include <stdint.h>
 
-#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
@@ -339,170 +339,93 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void m
   if (lane == 0) *g = 0;
 }
 
-// Scratch buffers of the exact path, one per (device, stream), grown on
-// demand.  The whole launch sequence runs under the lock, so a buffer is
-// never freed between the time a call takes it and the time its kernels are
-// enqueued; growth synchronises the stream before freeing the old buffer.
-// Entries are keyed by the stream handle: a stream destroyed with this work
-// still pending and a new stream given the same handle would share one.
-struct Scratch {
-  int dev;
-  hipStream_t s;
-  void *p;
-  size_t bytes;
-  // gate array of the last call, for the statistics
-  const int *gate;
-  int gate_ld, tiles_r, tiles_c;
+// Workspace layout (caller's memory, 256-byte aligned pieces): Xq (Rq x Kp
+// int8), Yq (Cq x Kp int8), nx (R ints), ny (Cn ints), gate ((Rq/64) x
+// (Cq/64) ints).
+struct WsLayout {
+  int Kp, Rq, Cq, gate_ld, gate_n;
+  size_t bx, by, bnx, bny, bgate;
+  size_t bytes() const { return bx + by + bnx + bny + bgate; }
 };
-std::mutex g_scratch_mu;
-std::vector<Scratch> g_scratch;
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-Scratch *scratch_for(int dev, hipStream_t s, size_t bytes) {
-  for (auto &e : g_scratch)
-    if (e.dev == dev && e.s == s) {
-      if (e.bytes >= bytes) return &e;
-      if (hipStreamSynchronize(s) != hipSuccess) return nullptr;
-      (void)hipFree(e.p);
-      e.p = nullptr;
-      e.bytes = 0;
-      e.gate = nullptr;
-      if (hipMalloc(&e.p, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        e.p = nullptr;
-        return nullptr;
-      }
-      e.bytes = bytes;
-      return &e;
-    }
-  void *p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    return nullptr;
-  }
-  g_scratch.push_back({dev, s, p, bytes, nullptr, 0, 0, 0});
-  return &g_scratch.back();
+WsLayout ws_layout(int R, int Cn, int K) {
+  WsLayout w;
+  w.Kp = (K + QK - 1) / QK * QK;
+  w.Rq = (R + QT - 1) / QT * QT;
+  w.Cq = (Cn + QT - 1) / QT * QT;
+  w.gate_ld = w.Cq / 64;
+  w.gate_n = (w.Rq / 64) * w.gate_ld;
+  w.bx = align256((size_t)w.Rq * w.Kp);
+  w.by = align256((size_t)w.Cq * w.Kp);
+  w.bnx = align256((size_t)R * 4);
+  w.bny = align256((size_t)Cn * 4);
+  w.bgate = align256((size_t)w.gate_n * 4);
+  return w;
 }
 
-// A call that takes the direct kernel on every tile: the statistics of this
-// (device, stream) then report all of its tiles as direct.  The _locked form
-// expects g_scratch_mu held.
-void record_direct_only_locked(int dev, hipStream_t s, int R, int Cn) {
-  for (auto &e : g_scratch)
-    if (e.dev == dev && e.s == s) {
-      e.gate = nullptr;
-      e.tiles_r = R > 0 ? (R + 63) / 64 : 0;
-      e.tiles_c = Cn > 0 ? (Cn + 63) / 64 : 0;
-    }
-}
-void record_direct_only(hipStream_t s, int R, int Cn) {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return;
-  std::lock_guard<std::mutex> lock(g_scratch_mu);
-  record_direct_only_locked(dev, s, R, Cn);
-}
+// K <= 65536 keeps every norm below 2^30 and every int32 dot product in range.
+bool exact_eligible(int R, int Cn, int K) { return R > 0 && Cn > 0 && K > 0 && K <= 65536; }
 
 }  // namespace
 
-hipError_t launch_minus_square(int R, int Cn, int K, const float *X, int64_t ldx,
-                               const float *Y, int64_t ldy, float *O, int64_t ldo,
-                               const float *C, int64_t ldc, hipStream_t s) {
+size_t minus_square_exact_workspace(int R, int Cn, int K) {
+  return exact_eligible(R, Cn, K) ? ws_layout(R, Cn, K).bytes() : 0;
+}
+
+hipError_t launch_minus_square_exact(int R, int Cn, int K, const float *X, int64_t ldx,
+                                     const float *Y, int64_t ldy, float *O, int64_t ldo,
+                                     const float *C, int64_t ldc, void *ws, size_t ws_bytes,
+                                     hipStream_t s) {
   Epilogue ep;
   ep.C = C;
   ep.ldc = ldc;
-  // K <= 65536 keeps every norm below 2^30 and every int32 dot product in
-  // range.
-  if (R <= 0 || Cn <= 0 || K <= 0 || K > 65536) {
-    record_direct_only(s, R, Cn);
+  if (!exact_eligible(R, Cn, K))
     return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
-  }
-  // Under graph capture the scratch buffer could be freed while a graph
-  // still refers to it: take the direct path there.
-  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
-    (void)hipGetLastError();
-    record_direct_only(s, R, Cn);
-    return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
-  }
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  const int Kp = (K + QK - 1) / QK * QK;
-  const int Rq = (R + QT - 1) / QT * QT, Cq = (Cn + QT - 1) / QT * QT;
-  const int gate_ld = Cq / 64, gate_n = (Rq / 64) * gate_ld;
-  const size_t bx = align256((size_t)Rq * Kp), by = align256((size_t)Cq * Kp);
-  const size_t bnx = align256((size_t)R * 4), bny = align256((size_t)Cn * 4);
-  const size_t bytes = bx + by + bnx + bny + align256((size_t)gate_n * 4);
-
-  std::lock_guard<std::mutex> lock(g_scratch_mu);
-  Scratch *sc = scratch_for(dev, s, bytes);
-  if (!sc) {
-    record_direct_only_locked(dev, s, R, Cn);
-    return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s);
-  }
-  char *base = static_cast<char *>(sc->p);
+  const WsLayout w = ws_layout(R, Cn, K);
+  if (!ws || ws_bytes < w.bytes()) return hipErrorInvalidValue;
+  char *base = static_cast<char *>(ws);
   int8_t *Xq = reinterpret_cast<int8_t *>(base);
-  int8_t *Yq = reinterpret_cast<int8_t *>(base + bx);
-  int *nx = reinterpret_cast<int *>(base + bx + by);
-  int *ny = reinterpret_cast<int *>(base + bx + by + bnx);
-  int *gate = reinterpret_cast<int *>(base + bx + by + bnx + bny);
+  int8_t *Yq = reinterpret_cast<int8_t *>(base + w.bx);
+  int *nx = reinterpret_cast<int *>(base + w.bx + w.by);
+  int *ny = reinterpret_cast<int *>(base + w.bx + w.by + w.bnx);
+  int *gate = reinterpret_cast<int *>(base + w.bx + w.by + w.bnx + w.bny);
 
-  const int qbx = Rq / QR, qby = Cq / QR;
-  msq_quantize_kernel<<<qbx + qby, 256, 0, s>>>(R, Cn, K, Kp, X, ldx, Y, ldy, Xq, Yq, nx, ny,
+  const int qbx = w.Rq / QR, qby = w.Cq / QR;
+  msq_quantize_kernel<<<qbx + qby, 256, 0, s>>>(R, Cn, K, w.Kp, X, ldx, Y, ldy, Xq, Yq, nx, ny,
                                                 qbx);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int tiles_r = w.Rq / QT, tiles_c = w.Cq / QT;
+  msq_mfma_kernel<<<tiles_r * tiles_c, 256, 0, s>>>(R, Cn, w.Kp, Xq, Yq, nx, ny, O, ldo, C, ldc,
+                                                    gate, w.gate_ld, tiles_r, tiles_c);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int tiles_r = Rq / QT, tiles_c = Cq / QT;
-  msq_mfma_kernel<<<tiles_r * tiles_c, 256, 0, s>>>(R, Cn, Kp, Xq, Yq, nx, ny, O, ldo, C, ldc,
-                                                    gate, gate_ld, tiles_r, tiles_c);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  sc->gate = gate;
-  sc->gate_ld = gate_ld;
-  sc->tiles_r = (R + 63) / 64;
-  sc->tiles_c = (Cn + 63) / 64;
   ep.gate = gate;
-  ep.gate_ld = gate_ld;
+  ep.gate_ld = w.gate_ld;
   return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s, CFG_S64);
 }
 
-hipError_t minus_square_tile_counts(hipStream_t s, int *mfma_tiles, int *valu_tiles) {
+hipError_t minus_square_exact_tile_counts(const void *ws, int R, int Cn, int K, hipStream_t s,
+                                          int *mfma_tiles, int *valu_tiles) {
   *mfma_tiles = 0;
   *valu_tiles = 0;
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  std::lock_guard<std::mutex> lock(g_scratch_mu);
-  for (auto &sc : g_scratch)
-    if (sc.dev == dev && sc.s == s && !sc.gate) {
-      *valu_tiles = sc.tiles_r * sc.tiles_c;
-      return hipSuccess;
-    } else if (sc.dev == dev && sc.s == s) {
-      std::vector<int> g((size_t)sc.tiles_r * sc.gate_ld);
-      e = hipStreamSynchronize(s);
-      if (e == hipSuccess)
-        e = hipMemcpy(g.data(), sc.gate, g.size() * sizeof(int), hipMemcpyDeviceToHost);
-      if (e != hipSuccess) return e;
-      for (int r = 0; r < sc.tiles_r; ++r)
-        for (int c = 0; c < sc.tiles_c; ++c) ++*(g[(size_t)r * sc.gate_ld + c] ? valu_tiles
-                                                                              : mfma_tiles);
-      return hipSuccess;
-    }
-  return hipSuccess;
-}
-
-hipError_t minus_square_release_scratch() {
-  std::lock_guard<std::mutex> lock(g_scratch_mu);
-  int cur = 0;
-  hipError_t e = hipGetDevice(&cur);
-  if (e != hipSuccess) return e;
-  for (auto &sc : g_scratch) {
-    if (hipSetDevice(sc.dev) == hipSuccess && hipDeviceSynchronize() == hipSuccess)
-      (void)hipFree(sc.p);
+  if (R <= 0 || Cn <= 0) return hipSuccess;
+  const int tr = (R + 63) / 64, tc = (Cn + 63) / 64;
+  if (!exact_eligible(R, Cn, K)) {
+    *valu_tiles = tr * tc;
+    return hipSuccess;
   }
-  g_scratch.clear();
-  return hipSetDevice(cur);
+  const WsLayout w = ws_layout(R, Cn, K);
+  const char *gate = static_cast<const char *>(ws) + w.bx + w.by + w.bnx + w.bny;
+  std::vector<int> g((size_t)tr * w.gate_ld);
+  hipError_t e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = hipMemcpy(g.data(), gate, g.size() * sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  for (int r = 0; r < tr; ++r)
+    for (int c = 0; c < tc; ++c) ++*(g[(size_t)r * w.gate_ld + c] ? valu_tiles : mfma_tiles);
+  return hipSuccess;
 }
 
 }  // namespace simd2

@@ -1,14 +1,15 @@
-"""minus_square's int8 matrix-core path against the oracle, bit for bit.
+"""simd2_minus_square_exact (minus_square on the int8 matrix cores) against
+the oracle, bit for bit.
 
 Tiles whose operands are integers in [-128, 127] and whose sums stay at or
 below 2^24 run on v_mfma_i32_16x16x64_i8 as |a|^2 + |b|^2 - 2 a.b; there the
 oracle's k-by-k FP32 sum is the same integer.  Every other tile runs the
-direct FP32 kernel.  The tests check the result against the oracle (or the
-direct kernel where the oracle would be slow) and, through
-simd2_minus_square_tiles, that each tile took the path its data calls for:
-fractional or out-of-range entries, infinities, and sums past 2^24 (where
-the oracle's additions round, so a tile written from the exact integer
-would differ)."""
+direct FP32 kernel (the one cuasr_plusmiussquare_srsgemm runs).  The tests
+check the result against the oracle (or the direct kernel where the oracle
+would be slow) and, through simd2_minus_square_tiles, that each tile took the
+path its data calls for: fractional or out-of-range entries, infinities, and
+sums past 2^24 (where the oracle's additions round, so a tile written from
+the exact integer would differ); and the caller-owned workspace's checks."""
 import sys
 
 import numpy as np
@@ -30,15 +31,22 @@ def tiles(M, N):
     return ((M + 63) // 64) * ((N + 63) // 64)
 
 
-def run(torch, A, B, C, epi, tile_config=None, stream=None):
+def run(torch, A, B, C, epi, direct=False, stream=None, workspace=None):
+    """D and the (int8, direct) tile counts of the exact entry point, or D of
+    the drop-in (the direct kernel) when `direct`."""
     import simd2_amd
     tA, tB, tC = (torch.from_numpy(np.ascontiguousarray(x)).to("cuda") for x in (A, B, C))
     out = torch.empty((A.shape[0], B.shape[1]), device="cuda")
     if stream is not None:
         stream.wait_stream(torch.cuda.current_stream())
-    D = simd2_amd.srgemm("minus_square", tA, tB, tC, epi, out=out, tile_config=tile_config,
-                         stream=stream)
-    counts = simd2_amd.minus_square_tiles(stream) if tile_config is None else None
+    counts = None
+    if direct:
+        D = simd2_amd.srgemm("minus_square", tA, tB, tC, epi, out=out, stream=stream)
+    else:
+        D, ws = simd2_amd.minus_square_exact(tA, tB, tC, epi, out=out, workspace=workspace,
+                                             stream=stream)
+        M, K = A.shape
+        counts = simd2_amd.minus_square_tiles(ws, M, B.shape[1], K, stream)
     torch.cuda.synchronize()
     return D.cpu().numpy(), counts
 
@@ -133,14 +141,15 @@ def test_fractional_data_takes_the_direct_kernel(gpu):
     B = rng.standard_normal((128, 320)).astype(np.float32)
     C = rng.standard_normal((384, 320)).astype(np.float32)
     D, (mf, va) = run(torch, A, B, C, True)
-    E, _ = run(torch, A, B, C, True, tile_config=1)
+    E, _ = run(torch, A, B, C, True, direct=True)
     assert np.array_equal(bits(D), bits(E))
     assert (mf, va) == (0, tiles(384, 320))
 
 
 def test_large_square_matches_direct_kernel(gpu):
     """2048^3 U{1..100} (the benchmark's data at half the size): the int8
-    path against the direct FP32 kernel, bit for bit, C aliased to D."""
+    path against the drop-in's direct FP32 kernel, bit for bit, C aliased to
+    D."""
     torch = gpu
     import simd2_amd
     g = torch.Generator(device="cuda")
@@ -151,23 +160,21 @@ def test_large_square_matches_direct_kernel(gpu):
     C = torch.randint(1, 101, (n, n), device="cuda", generator=g).float()
     say("int8 path 2048^3 vs direct kernel")
     D1 = C.clone()
-    simd2_amd.srgemm("minus_square", A, B, D1, True, out=D1)
-    mf, va = simd2_amd.minus_square_tiles()
-    D2 = simd2_amd.srgemm("minus_square", A, B, C, True, tile_config=1)
+    _, ws = simd2_amd.minus_square_exact(A, B, D1, True, out=D1)
+    mf, va = simd2_amd.minus_square_tiles(ws, n, n, n)
+    D2 = simd2_amd.srgemm("minus_square", A, B, C, True)
     torch.cuda.synchronize()
     assert torch.equal(D1.view(torch.int32), D2.view(torch.int32))
     assert (mf, va) == (tiles(n, n), 0)
 
 
 def test_k_past_the_int8_limit_takes_the_direct_kernel(gpu, oracle_mod):
-    """K = 70000 (past the 65536 the int8 path accepts) after an int8-path
-    call on the same stream: the result is the oracle's and every tile is
-    reported as direct."""
+    """K = 70000 (past the 65536 the int8 path accepts): no workspace is
+    needed, the result is the oracle's and every tile is reported direct."""
+    import simd2_amd
     rng = np.random.default_rng(23)
-    A, B, C = ints(rng, (64, 64), 1, 3), ints(rng, (64, 64), 1, 3), ints(rng, (64, 64), 1, 3)
-    D, (mf, va) = run(gpu, A, B, C, True)
-    assert (mf, va) == (1, 0)
     M, N, K = 3, 70, 70000
+    assert simd2_amd.minus_square_workspace(M, N, K) == 0
     A, B, C = ints(rng, (M, K), 0, 1), ints(rng, (K, N), 0, 1), ints(rng, (M, N), 0, 1)
     say("K = 70000, direct kernel")
     D, (mf, va) = run(gpu, A, B, C, True)
@@ -176,25 +183,37 @@ def test_k_past_the_int8_limit_takes_the_direct_kernel(gpu, oracle_mod):
     assert (mf, va) == (0, tiles(M, N)), (mf, va)
 
 
-def test_scratch_growth_streams_and_release(gpu, oracle_mod):
-    """Small, larger, small again on one stream (the buffer grows once), the
-    same on a second stream, then the buffers freed and used again."""
+def test_workspace_checks_and_reuse(gpu, oracle_mod):
+    """A missing or short workspace is refused with SIMD2_ERROR_WORKSPACE and
+    nothing is written; one workspace sized for the largest problem serves
+    smaller ones, on the default stream and on a second stream in turn."""
     torch = gpu
     import simd2_amd
     rng = np.random.default_rng(17)
+    M, N, K = 300, 200, 500
+    need = simd2_amd.minus_square_workspace(M, N, K)
+    assert need >= (M + N) * K
+    A, B, C = ints(rng, (M, K), 1, 100), ints(rng, (K, N), 1, 100), ints(rng, (M, N), 1, 100)
+    tA, tB, tC = (torch.from_numpy(x).cuda() for x in (A, B, C))
+    out = torch.full((M, N), 7.0, device="cuda")
+    short = torch.empty(need - 1, dtype=torch.uint8, device="cuda")
+    for ws in (short,):
+        with pytest.raises(simd2_amd.Simd2Error) as err:
+            simd2_amd.minus_square_exact(tA, tB, tC, True, out=out, workspace=ws)
+        assert err.value.status == simd2_amd.ERROR_WORKSPACE
+    st = simd2_amd.lib().simd2_minus_square_exact(N, M, K, tB.data_ptr(), N, tA.data_ptr(), K,
+                                                  tC.data_ptr(), N, out.data_ptr(), N, 1, None,
+                                                  need, torch.cuda.current_stream().cuda_stream)
+    assert st == simd2_amd.ERROR_WORKSPACE
+    torch.cuda.synchronize()
+    assert bool((out == 7.0).all())
+    big = torch.empty(need, dtype=torch.uint8, device="cuda")
     s2 = torch.cuda.Stream()
-    cases = [(100, 90, 80), (700, 600, 500), (64, 64, 64)]
     for stream in (None, s2):
-        for M, N, K in cases:
-            A, B, C = ints(rng, (M, K), 1, 100), ints(rng, (K, N), 1, 100), ints(rng, (M, N), 1, 100)
-            say(f"scratch {M}x{N}x{K} stream={'s2' if stream else 'default'}")
-            D, (mf, va) = run(torch, A, B, C, True, stream=stream)
+        for m, n, k in ((M, N, K), (100, 90, 80), (64, 64, 64)):
+            A, B, C = ints(rng, (m, k), 1, 100), ints(rng, (k, n), 1, 100), ints(rng, (m, n), 1, 100)
+            say(f"workspace reuse {m}x{n}x{k} stream={'s2' if stream else 'default'}")
+            D, (mf, va) = run(torch, A, B, C, True, stream=stream, workspace=big)
             want = oracle_mod.srgemm_f32("minus_square", A, B, C, True, threads=8)
             assert np.array_equal(bits(D), bits(want))
-            assert (mf, va) == (tiles(M, N), 0)
-        simd2_amd.release_scratch()
-    M, N, K = cases[1]
-    A, B, C = ints(rng, (M, K), 1, 100), ints(rng, (K, N), 1, 100), ints(rng, (M, N), 1, 100)
-    D, (mf, va) = run(torch, A, B, C, False)
-    want = oracle_mod.srgemm_f32("minus_square", A, B, C, False, threads=8)
-    assert np.array_equal(bits(D), bits(want)) and mf == tiles(M, N)
+            assert (mf, va) == (tiles(m, n), 0)
