@@ -217,3 +217,36 @@ def test_workspace_checks_and_reuse(gpu, oracle_mod):
             want = oracle_mod.srgemm_f32("minus_square", A, B, C, True, threads=8)
             assert np.array_equal(bits(D), bits(want))
             assert (mf, va) == (tiles(m, n), 0)
+
+
+def test_captured_into_a_graph(gpu):
+    """The entry point allocates nothing, so it can be captured: three replays
+    of a captured call, with eager work between them, each equal to the eager
+    result (inputs changed in place between replays)."""
+    torch = gpu
+    import simd2_amd
+    g = torch.Generator(device="cuda")
+    g.manual_seed(31)
+    M, N, K = 384, 320, 700
+    A = torch.randint(1, 101, (M, K), device="cuda", generator=g).float()
+    B = torch.randint(1, 101, (K, N), device="cuda", generator=g).float()
+    C = torch.randint(1, 101, (M, N), device="cuda", generator=g).float()
+    D = torch.empty((M, N), device="cuda")
+    ws = torch.empty(simd2_amd.minus_square_workspace(M, N, K), dtype=torch.uint8, device="cuda")
+    simd2_amd.minus_square_exact(A, B, C, True, out=D, workspace=ws)  # warm-up
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        simd2_amd.minus_square_exact(A, B, C, True, out=D, workspace=ws)
+    for rep in range(3):
+        A.copy_(torch.randint(1, 101, (M, K), device="cuda", generator=g).float())
+        C.mul_(1.0).add_(rep)
+        torch.cuda.synchronize()
+        say(f"graph replay {rep}")
+        graph.replay()
+        torch.cuda.synchronize()
+        got = D.clone()
+        want = simd2_amd.srgemm("minus_square", A, B, C, True)
+        torch.cuda.synchronize()
+        assert torch.equal(got.view(torch.int32), want.view(torch.int32)), rep
+        assert simd2_amd.minus_square_tiles(ws, M, N, K) == (tiles(M, N), 0)
