@@ -173,7 +173,8 @@ int simd2_orand_packed(int M, int N, int K, const uint32_t *A, int lda_words,
  * Other tiles run the direct FP32 kernel.  `workspace` (device memory, kept
  * by the caller) must hold simd2_minus_square_workspace(M, N, K) bytes, about
  * (M + N) * K; that size is 0 when K > 65536, and then every tile runs the
- * direct kernel.  Calls sharing a workspace must be ordered (one stream). */
+ * direct kernel.  Calls sharing a workspace must be ordered (one stream).
+ * A workspace overlapping A, B, C or D is refused (SIMD2_ERROR_ALIASING). */
 size_t simd2_minus_square_workspace(int M, int N, int K);
 int simd2_minus_square_exact(int M, int N, int K, const float *A, int lda, const float *B,
                              int ldb, const float *C, int ldc, float *D, int ldd,

@@ -233,6 +233,13 @@ int simd2_minus_square_exact(int M, int N, int K, const float *A, int lda, const
   if (st != SIMD2_SUCCESS) return st;
   const size_t need = simd2_minus_square_workspace(M, N, K);
   if (need > 0 && (!workspace || workspace_bytes < need)) return SIMD2_ERROR_WORKSPACE;
+  if (need > 0) {  // the workspace is written: it may not overlap an operand
+    const uintptr_t w0 = reinterpret_cast<uintptr_t>(workspace);
+    const std::pair<uintptr_t, uintptr_t> w{w0, w0 + need};
+    if (overlap(w, span(A, M, K, lda, 4)) || overlap(w, span(B, K, N, ldb, 4)) ||
+        overlap(w, span(D, M, N, ldd, 4)) || (epi && overlap(w, span(C, M, N, ldc, 4))))
+      return SIMD2_ERROR_ALIASING;
+  }
   return hip_status(launch_minus_square_exact(N, M, K, B, ldb, A, lda, D, ldd,
                                               epi ? C : nullptr, ldc, workspace,
                                               workspace_bytes, stream));

@@ -205,8 +205,15 @@ def test_workspace_checks_and_reuse(gpu, oracle_mod):
                                                   tC.data_ptr(), N, out.data_ptr(), N, 1, None,
                                                   need, torch.cuda.current_stream().cuda_stream)
     assert st == simd2_amd.ERROR_WORKSPACE
+    # a workspace inside the output buffer
+    big_out = torch.full((M + need // (4 * N) + 2, N), 7.0, device="cuda")
+    st = simd2_amd.lib().simd2_minus_square_exact(N, M, K, tB.data_ptr(), N, tA.data_ptr(), K,
+                                                  tC.data_ptr(), N, big_out.data_ptr(), N, 1,
+                                                  big_out.data_ptr() + 4 * N * 2, need,
+                                                  torch.cuda.current_stream().cuda_stream)
+    assert st == simd2_amd.ERROR_ALIASING
     torch.cuda.synchronize()
-    assert bool((out == 7.0).all())
+    assert bool((out == 7.0).all()) and bool((big_out == 7.0).all())
     big = torch.empty(need, dtype=torch.uint8, device="cuda")
     s2 = torch.cuda.Stream()
     for stream in (None, s2):
