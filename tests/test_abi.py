@@ -90,6 +90,45 @@ def test_generic_entry_point_errors(lib):
     assert g(4, 40, 40, P1, 2, P2, 2, P3, 2, P1, 2, 1, None) == simd2_amd.ERROR_ALIASING
 
 
+def test_minus_square_exact_workspace_and_argument_errors(lib):
+    """The workspace size (int8 panels padded to 128 rows / columns and to a
+    multiple of 64 in k, norms, tile gates, each 256-byte aligned) and the
+    host-side refusals, none of which reaches the GPU."""
+    f = lib.simd2_minus_square_workspace
+    a256 = lambda b: (b + 255) // 256 * 256
+
+    def want(M, N, K):  # column-major M x N: the row-major N x M problem
+        R, Cn, Kp = N, M, (K + 63) // 64 * 64
+        Rq, Cq = (R + 127) // 128 * 128, (Cn + 127) // 128 * 128
+        return (a256(Rq * Kp) + a256(Cq * Kp) + a256(4 * R) + a256(4 * Cn) +
+                a256(4 * (Rq // 64) * (Cq // 64)))
+    for M, N, K in ((300, 200, 500), (1, 1, 1), (128, 129, 64), (4096, 4096, 4096),
+                    (5, 7, 65536)):
+        assert f(M, N, K) == want(M, N, K), (M, N, K)
+    assert f(300, 200, 65537) == 0 and f(0, 4, 4) == 0 and f(4, 4, 0) == 0
+    g = lib.simd2_minus_square_exact
+    need = f(4, 4, 4)
+    W = 16 << 20
+    assert g(4, 4, 4, P1, 4, P2, 4, P3, 4, P4, 4, 1, None, need, None) == \
+        simd2_amd.ERROR_WORKSPACE
+    assert g(4, 4, 4, P1, 4, P2, 4, P3, 4, P4, 4, 1, W, need - 1, None) == \
+        simd2_amd.ERROR_WORKSPACE
+    assert g(4, 4, 4, P1, 4, P2, 4, P3, 4, P4, 4, 1, P4 - 8, need, None) == \
+        simd2_amd.ERROR_ALIASING
+    assert g(4, 4, 4, P1, 4, P2, 4, P3, 4, P4, 4, 1, P1 + 4, need, None) == \
+        simd2_amd.ERROR_ALIASING
+    assert g(-1, 4, 4, P1, 4, P2, 4, P3, 4, P4, 4, 1, W, need, None) == \
+        simd2_amd.ERROR_INVALID_PROBLEM
+    assert g(4, 4, 4, None, 4, P2, 4, P3, 4, P4, 4, 1, W, need, None) == \
+        simd2_amd.ERROR_NULL_POINTER
+    assert g(0, 4, 4, P1, 4, P2, 4, P3, 4, P4, 4, 1, None, 0, None) == simd2_amd.SUCCESS
+    # K past the int8 limit: all tiles are direct, and the count needs no GPU
+    mf, va = ctypes.c_int(-1), ctypes.c_int(-1)
+    assert lib.simd2_minus_square_tiles(None, 100, 130, 70000, None, ctypes.byref(mf),
+                                        ctypes.byref(va)) == simd2_amd.SUCCESS
+    assert (mf.value, va.value) == (0, 2 * 3)
+
+
 def test_converge_argument_errors(lib):
     itr = ctypes.c_int(-5)
     f = lib.simd2_converge_f32
