@@ -49,7 +49,10 @@ namespace {
 // ---------------------------------------------------------------------------
 constexpr int SEL_KMAX = 32;
 constexpr int SEL_Q = 64;      // queries per workgroup (one per lane)
-constexpr int SEL_PARTS = 2;   // waves splitting the references
+constexpr int SEL_U = 16;      // distance loads in flight per lane
+// Waves splitting the references: as many as the merge lists fit in LDS
+// (PARTS x KMAX x 64 x 8 bytes).
+template <int KMAX> constexpr int sel_parts() { return KMAX <= 16 ? 16 : 8; }
 
 // Stable insertion into a sorted register list: strictly smaller distances
 // move ahead, equal ones stay behind (earlier index first); while the list
@@ -86,7 +89,7 @@ __device__ __forceinline__ void sorted_insert(float (&ld)[KMAX], int (&li)[KMAX]
   thr = last;
 }
 
-template <int KMAX>
+template <int KMAX, int SEL_PARTS = sel_parts<KMAX>()>
 __global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
     const float *__restrict__ dist, int Q, int R, int k, float *out_d, int *out_i, int do_sqrt) {
   __shared__ float sd[SEL_PARTS][KMAX][SEL_Q];
@@ -109,12 +112,12 @@ __global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
   float thr = FLT_MAX;
   if (q < Q) {
     int r = r0;
-    for (; r + 8 <= r1; r += 8) {
-      float v[8];
+    for (; r + SEL_U <= r1; r += SEL_U) {
+      float v[SEL_U];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = dist[(int64_t)(r + u) * Q + q];
+      for (int u = 0; u < SEL_U; ++u) v[u] = dist[(int64_t)(r + u) * Q + q];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) sorted_insert<KMAX>(ld, li, n, thr, k, v[u], r + u);
+      for (int u = 0; u < SEL_U; ++u) sorted_insert<KMAX>(ld, li, n, thr, k, v[u], r + u);
     }
     for (; r < r1; ++r) sorted_insert<KMAX>(ld, li, n, thr, k, dist[(int64_t)r * Q + q], r);
   }
@@ -126,19 +129,33 @@ __global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
   sn[part][lane] = n;
   __syncthreads();
   if (part != 0 || q >= Q) return;
-  // Merge: part 0 holds the lower indices, so on equal distances it wins,
-  // which keeps index order.
-  int p0 = 0, p1 = 0;
-  const int n0 = sn[0][lane], n1 = sn[1][lane];
+  // Merge of the parts' sorted lists: part p holds lower indices than part
+  // p + 1, so on equal distances the lowest part wins, which keeps index
+  // order.
+  int pos[SEL_PARTS];
+#pragma unroll
+  for (int p = 0; p < SEL_PARTS; ++p) pos[p] = 0;
   for (int j = 0; j < k; ++j) {
-    const bool has0 = p0 < n0, has1 = p1 < n1;
-    if (!has0 && !has1) break;
-    const float d0 = has0 ? sd[0][p0][lane] : 0.f;
-    const float d1 = has1 ? sd[1][p1][lane] : 0.f;
-    const bool take0 = has0 && (!has1 || !(d1 < d0));
-    const float bd = take0 ? d0 : d1;
-    const int bi = take0 ? si[0][p0][lane] : si[1][p1][lane];
-    if (take0) ++p0; else ++p1;
+    int best = -1;
+    float bd = 0.f;
+#pragma unroll
+    for (int p = 0; p < SEL_PARTS; ++p) {
+      if (pos[p] < sn[p][lane]) {
+        const float dp = sd[p][pos[p]][lane];
+        if (best < 0 || dp < bd) {
+          best = p;
+          bd = dp;
+        }
+      }
+    }
+    if (best < 0) break;
+    int bi = 0;
+#pragma unroll
+    for (int p = 0; p < SEL_PARTS; ++p)
+      if (p == best) {
+        bi = si[p][pos[p]][lane];
+        ++pos[p];
+      }
     out_d[(int64_t)j * Q + q] = do_sqrt ? sqrtf(bd) : bd;
     out_i[(int64_t)j * Q + q] = bi;
   }
@@ -592,11 +609,13 @@ int dpad_of(int d) { return ((d + 31) / 32) * 32; }
 
 hipError_t launch_knn_select(const float *dist, int Q, int R, int k, float *out_d, int *out_i,
                              int do_sqrt, hipStream_t s) {
-  const dim3 grid((Q + SEL_Q - 1) / SEL_Q), block(SEL_Q * SEL_PARTS);
+  const dim3 grid((Q + SEL_Q - 1) / SEL_Q);
   if (k <= 16)
-    knn_select_kernel<16><<<grid, block, 0, s>>>(dist, Q, R, k, out_d, out_i, do_sqrt);
+    knn_select_kernel<16><<<grid, SEL_Q * sel_parts<16>(), 0, s>>>(dist, Q, R, k, out_d, out_i,
+                                                                   do_sqrt);
   else
-    knn_select_kernel<SEL_KMAX><<<grid, block, 0, s>>>(dist, Q, R, k, out_d, out_i, do_sqrt);
+    knn_select_kernel<SEL_KMAX><<<grid, SEL_Q * sel_parts<SEL_KMAX>(), 0, s>>>(
+        dist, Q, R, k, out_d, out_i, do_sqrt);
   return hipGetLastError();
 }
 
