@@ -54,39 +54,46 @@ constexpr int SEL_U = 16;      // distance loads in flight per lane
 // (PARTS x KMAX x 64 x 8 bytes).
 template <int KMAX> constexpr int sel_parts() { return KMAX <= 16 ? 16 : 8; }
 
-// Stable insertion into a sorted register list: strictly smaller distances
-// move ahead, equal ones stay behind (earlier index first); while the list
-// holds fewer than k entries every element enters.  `thr` caches entry k-1
-// so no register is indexed dynamically.
+// The reference's top-k (apps/pld/cuASR/knncuda.cu:170-237) keeps a sorted
+// list: while it holds fewer than k entries every element enters; then an
+// element enters only with a distance strictly below the k-th, equal
+// distances stay in index order, and the last entry drops out.  Here the list
+// is kept unsorted with its largest entry cached: an element enters under the
+// same rule and replaces the entry that is largest by (distance, index) --
+// the one the sorted list drops -- and the k entries are sorted by (distance,
+// index) at the end, which gives the same list.  An insertion rescans the k
+// entries for the new maximum instead of shifting them.
 template <int KMAX>
-__device__ __forceinline__ void sorted_insert(float (&ld)[KMAX], int (&li)[KMAX], int &n,
-                                              float &thr, int k, float d, int idx) {
-  if (n >= k && !(d < thr)) return;
-  float cd = d;
-  int ci = idx;
-  float last = thr;
-  // The new element goes to the first position holding a strictly larger
-  // distance (or the first free one); every entry after it then shifts one
-  // place, unconditionally, so equal distances keep their order.
-  bool shifting = false;
+__device__ __forceinline__ void max_insert(float (&ld)[KMAX], int (&li)[KMAX], int &n,
+                                           float &thr, int &tpos, int k, float d, int idx) {
+  if (n < k) {
 #pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    if (j < k) {
-      const bool sw = shifting || ((j < n) ? (cd < ld[j]) : (j == n));
-      const float td = ld[j];
-      const int ti = li[j];
-      if (sw) {
-        ld[j] = cd;
-        li[j] = ci;
-        cd = td;
-        ci = ti;
-        shifting = true;
+    for (int j = 0; j < KMAX; ++j)
+      if (j == n) {
+        ld[j] = d;
+        li[j] = idx;
       }
-      if (j == k - 1) last = ld[j];
-    }
+    if (++n < k) return;
+  } else {
+    if (!(d < thr)) return;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+      if (j == tpos) {
+        ld[j] = d;
+        li[j] = idx;
+      }
   }
-  if (n < k) ++n;
-  thr = last;
+  float md = ld[0];
+  int mi = li[0], mp = 0;
+#pragma unroll
+  for (int j = 1; j < KMAX; ++j)
+    if (j < k && (ld[j] > md || (ld[j] == md && li[j] > mi))) {
+      md = ld[j];
+      mi = li[j];
+      mp = j;
+    }
+  thr = md;
+  tpos = mp;
 }
 
 template <int KMAX, int SEL_PARTS = sel_parts<KMAX>()>
@@ -104,12 +111,12 @@ __global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
   float ld[KMAX];
   int li[KMAX];
 #pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    ld[j] = FLT_MAX;
-    li[j] = -1;
+  for (int j = 0; j < KMAX; ++j) {  // unfilled entries sort last
+    ld[j] = __builtin_inff();
+    li[j] = 0x7fffffff;
   }
-  int n = 0;
-  float thr = FLT_MAX;
+  int n = 0, tpos = 0;
+  float thr = __builtin_inff();
   if (q < Q) {
     int r = r0;
     for (; r + SEL_U <= r1; r += SEL_U) {
@@ -117,10 +124,23 @@ __global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
 #pragma unroll
       for (int u = 0; u < SEL_U; ++u) v[u] = dist[(int64_t)(r + u) * Q + q];
 #pragma unroll
-      for (int u = 0; u < SEL_U; ++u) sorted_insert<KMAX>(ld, li, n, thr, k, v[u], r + u);
+      for (int u = 0; u < SEL_U; ++u) max_insert<KMAX>(ld, li, n, thr, tpos, k, v[u], r + u);
     }
-    for (; r < r1; ++r) sorted_insert<KMAX>(ld, li, n, thr, k, dist[(int64_t)r * Q + q], r);
+    for (; r < r1; ++r) max_insert<KMAX>(ld, li, n, thr, tpos, k, dist[(int64_t)r * Q + q], r);
   }
+  // sort by (distance, index): odd-even transposition over the registers
+#pragma unroll
+  for (int pass = 0; pass < KMAX; ++pass)
+#pragma unroll
+    for (int j = pass & 1; j + 1 < KMAX; j += 2) {
+      const bool sw = ld[j + 1] < ld[j] || (ld[j + 1] == ld[j] && li[j + 1] < li[j]);
+      const float td = sw ? ld[j + 1] : ld[j];
+      const int ti = sw ? li[j + 1] : li[j];
+      ld[j + 1] = sw ? ld[j] : ld[j + 1];
+      li[j + 1] = sw ? li[j] : li[j + 1];
+      ld[j] = td;
+      li[j] = ti;
+    }
 #pragma unroll
   for (int j = 0; j < KMAX; ++j) {
     sd[part][j][lane] = ld[j];
