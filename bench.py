@@ -564,7 +564,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"srgemm M=N=K={n}, all nine semirings (one launch each per step)",
+        "config": {"workload": f"srgemm M=N=K={n}, all nine semirings, one call each per step "
+                               "(the drop-ins; or_and through simd2_orand_packed, minus_square "
+                               "through simd2_minus_square_exact with its workspace)",
                    "M": n, "N": n, "K": n, "epilogue": True,
                    "or_and": "bit-packed uint32, density 0.02"},
         "per_op": per_op,
