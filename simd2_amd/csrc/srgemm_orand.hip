@@ -192,33 +192,47 @@ __global__ __launch_bounds__(256) void orand_i32_kernel(
     for (int j = 0; j < 4; ++j) acc[i][j] = 0u;
 
   for (int k0 = 0; k0 < K; k0 += IB_BKW * 32) {
-    // X: each wave packs 16 rows; per row, 4 ballots of 64 consecutive k.
-    for (int rr = 0; rr < IB_BM / 4; ++rr) {
-      const int lr = wave * (IB_BM / 4) + rr;
-      const int r = row0 + lr;
+    // Loads are issued in batches before any is used (addresses clamped into
+    // the matrix, values outside it taken as false), so a batch waits for
+    // memory once.
+    // X: each wave packs 16 rows, 4 at a time; per row, 4 ballots of 64
+    // consecutive k.
+    for (int rr = 0; rr < IB_BM / 4; rr += 4) {
+      int v[4][IB_BKW / 2];
 #pragma unroll
-      for (int ch = 0; ch < IB_BKW / 2; ++ch) {
-        const int k = k0 + ch * 64 + lane;
-        const bool bit = (r < R && k < K) ? (X[(int64_t)r * ldx + k] != 0) : false;
-        const uint64_t m = __ballot(bit);
-        if (lane == 0) {
-          Xp[lr][ch * 2] = (uint32_t)m;
-          Xp[lr][ch * 2 + 1] = (uint32_t)(m >> 32);
+      for (int i = 0; i < 4; ++i) {
+        const int r = min(row0 + wave * (IB_BM / 4) + rr + i, R - 1);
+#pragma unroll
+        for (int ch = 0; ch < IB_BKW / 2; ++ch)
+          v[i][ch] = X[(int64_t)r * ldx + min(k0 + ch * 64 + lane, K - 1)];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int lr = wave * (IB_BM / 4) + rr + i;
+#pragma unroll
+        for (int ch = 0; ch < IB_BKW / 2; ++ch) {
+          const int k = k0 + ch * 64 + lane;
+          const uint64_t m = __ballot(row0 + lr < R && k < K && v[i][ch] != 0);
+          if (lane == 0) {
+            Xp[lr][ch * 2] = (uint32_t)m;
+            Xp[lr][ch * 2 + 1] = (uint32_t)(m >> 32);
+          }
         }
       }
     }
     // Y: lane = column, each wave walks 64 k (two words) of its quarter.
     {
-      const int c = col0 + lane;
+      const int c = col0 + lane, cc = min(c, Cn - 1);
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
-        uint32_t wv = 0;
         const int kb = k0 + wave * 64 + half * 32;
-        for (int q = 0; q < 32; ++q) {
-          const int k = kb + q;
-          const bool bit = (c < Cn && k < K) ? (Y[(int64_t)k * ldy + c] != 0) : false;
-          wv |= (uint32_t)bit << q;
-        }
+        int v[32];
+#pragma unroll
+        for (int q = 0; q < 32; ++q) v[q] = Y[(int64_t)min(kb + q, K - 1) * ldy + cc];
+        uint32_t wv = 0;
+#pragma unroll
+        for (int q = 0; q < 32; ++q)
+          wv |= (uint32_t)(c < Cn && kb + q < K && v[q] != 0) << q;
         Yp[lane][wave * 2 + half] = wv;
       }
     }
