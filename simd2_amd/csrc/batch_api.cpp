@@ -56,6 +56,9 @@ struct simd2_batch {
   size_t launch_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Recorded after every solve: the tables and the pool are replaced or freed
+  // only after the last solve, on whatever stream it ran, has finished.
+  hipEvent_t solved = nullptr;
 
   int *active() { return d_state; }
   int *changed() { return d_state + count; }
@@ -212,7 +215,13 @@ hipError_t launch_round(simd2_batch &b, int r, hipStream_t s) {
   return e;
 }
 
+// Waits for the last solve (any stream) before its tables change.
+void wait_solved(simd2_batch &b) {
+  if (b.solved) (void)hipEventSynchronize(b.solved);
+}
+
 void destroy_graph(simd2_batch &b) {
+  wait_solved(b);
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
   if (b.graph) (void)hipGraphDestroy(b.graph);
   b.exec = nullptr;
@@ -363,6 +372,7 @@ int simd2_batch_create(const simd2_batch_graph *graphs, int count, int max_round
 int simd2_batch_destroy(simd2_batch *b) {
   if (!b) return SIMD2_SUCCESS;
   destroy_graph(*b);
+  if (b->solved) (void)hipEventDestroy(b->solved);
   if (b->pool) (void)hipFree(b->pool);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_state) (void)hipFree(b->d_state);
@@ -410,7 +420,16 @@ int simd2_batch_solve(simd2_batch *b, hipStream_t stream) {
   if (!b) return SIMD2_ERROR_NULL_POINTER;
   if (b->count == 0) return SIMD2_SUCCESS;
   if (!b->exec) return SIMD2_ERROR_INTERNAL;
-  return hip_status(hipGraphLaunch(b->exec, stream));
+  if (!b->solved) {
+    const hipError_t e = hipEventCreateWithFlags(&b->solved, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      b->solved = nullptr;
+      return hip_status(e);
+    }
+  }
+  hipError_t e = hipGraphLaunch(b->exec, stream);
+  if (e == hipSuccess) e = hipEventRecord(b->solved, stream);
+  return hip_status(e);
 }
 
 int simd2_batch_iterations(simd2_batch *b, int *iters, int *unconverged, hipStream_t stream) {

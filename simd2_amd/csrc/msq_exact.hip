@@ -326,12 +326,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void m
     for (int v = 0; v < 4; ++v) {
       const int r = qr0 + i * 16 + (lane / 16) * 4 + v;
       if (r >= R) continue;
-      const int nxv = nx[r];
+      const uint32_t nxv = (uint32_t)nx[r];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int c = qc0 + j * 16 + fr;
         if (c >= Cn) continue;
-        float d = (float)(nxv + nyv[j] - 2 * acc[i][j][v]);  // 0 <= s <= 2^24
+        // the gate loop's exact unsigned sum, here known to be <= 2^24
+        float d = (float)(nxv + (uint32_t)nyv[j] - 2u * (uint32_t)acc[i][j][v]);
         if (C) d = d + C[(int64_t)r * ldc + c];
         O[(int64_t)r * ldo + c] = d;
       }

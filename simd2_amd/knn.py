@@ -90,6 +90,14 @@ class KnnBF16:
 def knn_bf16(query, ref, k, sqrt=False, stream=None):
     """One-shot precision-form kNN.  Returns (dist Q x k, index Q x k, stats)
     where stats = [queries that used FP64, candidates recomputed in FP64]."""
+    torch = _torch()
     runner = KnnBF16(query.shape[0], ref.shape[0], query.shape[1], device=query.device)
     d, i = runner(query, ref, k, sqrt=sqrt, stream=stream)
+    if stream is not None:
+        # The workspace and outputs were allocated on the current stream but
+        # are used by kernels on `stream`: keep the caching allocator from
+        # reusing them before that stream's work is done.
+        s = stream if isinstance(stream, torch.cuda.Stream) else torch.cuda.ExternalStream(stream)
+        for t in (runner.ws, runner.stats, d, i):
+            t.record_stream(s)
     return d, i, runner.stats
