@@ -82,10 +82,14 @@ hipError_t launch_tensor_emu(int op, const void *a, const void *b, const float *
 hipError_t launch_f2h(void *out, const float *in, int64_t n, hipStream_t s);
 hipError_t launch_h2f(float *out, const void *in, int64_t n, hipStream_t s);
 
-// k-nearest neighbours (knn.hip).
+// Top-k of a materialised distance matrix (knn_select.hip): any 1 <= k <= R,
+// the workspace holds knn_select_workspace(Q, R, k) bytes.
+size_t knn_select_workspace(int Q, int R, int k);
 hipError_t launch_knn_select(const float *dist, int Q, int R, int k, float *out_d, int *out_i,
-                             int do_sqrt, hipStream_t s);
-size_t knn_bf16_workspace(int Q, int R, int d, int *slices_out);
+                             int do_sqrt, void *workspace, hipStream_t s);
+// k-nearest neighbours, precision form (knn.hip).
+size_t knn_bf16_workspace(int Q, int R, int d);
+int knn_bf16_kmax();
 hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, int64_t ldr, int Q,
                            int R, int d, int k, float *out_d, int *out_i, int do_sqrt,
                            void *workspace, int *stats, hipStream_t s);

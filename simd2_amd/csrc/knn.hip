@@ -1,39 +1,57 @@
 // k-nearest neighbours by pairwise squared L2 -- the PLD app's use of the
-// minus_square semiring (apps/pld/cuASR/knncuda.cu:399-549) -- on gfx950.
+// minus_square semiring (apps/pld/cuASR/knncuda.cu:399-549) -- in the
+// precision form (no counterpart in the reference; the reference's top-k of
+// a materialised distance matrix is knn_select.hip).
 //
-// 1. knn_select_kernel: the reference's post-processing of a materialised
-//    distance matrix (modified_insertion_sort_linear + compute_sqrt_linear,
-//    knncuda.cu:170-237): per query the k smallest (distance, index) pairs in
-//    ascending order, ties kept in index order, optionally square-rooted.
-//    Layout as the reference: dist[q + r * Q], outputs [j * Q + q].  Two waves
-//    scan two halves of the references for the same 64 queries (coalesced
-//    rows), each keeping a sorted register list, and merge through LDS.
-//    k <= 32.
+// Candidates come from ||q||^2 + ||r||^2 - 2 q.r with the dot product on BF16
+// MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate), the answer from exact
+// FP32 distances of those candidates, FP64 where FP32 cannot order them:
 //
-// 2. The precision form (no counterpart in the reference):
-//    * prep_kernel: FP32 points -> BF16 high and low parts (zero-padded to a
-//      multiple of 32 dims) and FP32 squared norms;
-//    * knn_pass_kernel: ||q||^2 + ||r||^2 - 2 q.r with the dot product on BF16
-//      MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate; hi.hi + hi.lo + lo.hi,
-//      three MFMAs per k-step, for ~16 significant bits).  A workgroup owns 128
-//      queries (each wave 32, their A fragments held in registers for the whole
-//      scan) and one slice of the references, streamed through LDS in 64-row
-//      tiles; the distances stay in the MFMA accumulators (lane l: 4 queries x
-//      4 references of a tile).  Two passes over the same products:
-//        pass 1 writes, per (query, tile), the smallest distance of the tile;
-//        knn_tau_kernel takes each query's CAND_K-th smallest tile minimum,
-//          tau: CAND_K distinct references lie at or under it, so it bounds
-//          the query's CAND_K-th best distance from above;
-//        pass 2 appends every reference at or under tau to the query's
-//          candidate buffer (rare: about CAND_K to 2 CAND_K per query).
-//      Neither pass keeps lists in the inner loop, and the Q x R matrix is
-//      never written;
-//    * rerank_kernel: one wave per query picks the 32 best candidates by the
-//      approximate distance, recomputes them exactly in FP32 ((q - r)^2 summed
-//      by the wave), ranks by (distance, index), and for candidates whose FP32
-//      distance lies within a relative 1e-5 of another one near the top k --
-//      where cancellation or rounding could order them wrongly -- recomputes the
-//      distance in FP64 and ranks those by it.  FP64 is used nowhere else.
+//  1. centre: mu = the references' mean (per dimension, fixed-order double
+//     sums).  Distances do not change under a common translation, and
+//     centring keeps ||q - mu||^2 and ||r - mu||^2 -- which the expansion's
+//     error scales with -- near the data's own spread even for data far from
+//     the origin;
+//  2. prep: centred points split into BF16 high and low parts (x - hi), dims
+//     zero-padded to a multiple of 32, and their FP32 squared norms; the
+//     largest reference norm;
+//  3. pass 1: per (query, 64-reference tile) the smallest approximate
+//     distance.  q.r ~ qh.rh + qh.rl + ql.rh, three MFMAs per 32-dim step,
+//     ~16 significant bits.  Up to 160 dims a workgroup holds its 128
+//     queries' fragments in registers and streams references through LDS;
+//     beyond, queries and references are both streamed through LDS in 32-dim
+//     slabs.  The Q x R matrix is never written;
+//  4. tau: per query the CK-th smallest tile minimum (CK = k + 16): CK
+//     distinct references have approximate distance <= tau.  With E_q a bound
+//     on |approximate - exact| for the query (below), their exact distances
+//     are <= tau + E_q, so the k-th nearest's is too, and every one of the k
+//     nearest has approximate distance <= thr = tau + 2 E_q;
+//  5. pass 2: the same products; every reference with approximate distance
+//     <= thr is appended to the query's candidate list (LDS-staged, one
+//     global atomic per query and workgroup).  A query whose list overflows
+//     (only when more than CAND_CAP references crowd within 2 E_q of its
+//     neighbours, e.g. thousands of duplicate points) is listed;
+//  6. overflow (listed queries only): the exact FP32 distance to every
+//     reference, and the CAND_CAP smallest (distance, index) keys become the
+//     query's candidates;
+//  7. re-rank, one wave per query: exact FP32 distance of every candidate
+//     (eight fixed partial sums over dims j = p mod 8, combined in a fixed
+//     tree -- the same value wherever it is computed), sort by (distance,
+//     index); runs of candidates whose FP32 distances lie within a relative
+//     tol of their neighbour and that reach rank k are recomputed in FP64 and
+//     ordered by it.  FP64 is used nowhere else.
+//
+// E_q.  With x = hi + lo + e, |e| <= 2^-16 |x| (two round-to-nearest BF16
+// steps of unit 2^-8), each product the three MFMAs drop (lo.lo and the e
+// terms) is below 3.1 2^-16 |x_j y_j|; the FP32 accumulation of the 3 dpad
+// products rounds at most 3 dpad times, each by 2^-24 of the running sum of
+// magnitudes; so the dot product errs by at most
+// (3.1 2^-16 + 3 dpad 2^-24) sum |x_j y_j| <= that / 2 (|x|^2 + |y|^2), and
+// the distance (twice the dot) by that times (|x|^2 + |y|^2).  The FP32 norms
+// (per-lane sums of d/64 squares and a 6-level tree) and the final two
+// additions add (d/64 + 8) 2^-24 (|x|^2 + |y|^2).  E_q = 1.25 x the sum of
+// these coefficients x (|q - mu|^2 + max_r |r - mu|^2) bounds the error of
+// every approximate distance of query q.
 #include <float.h>
 #include <hip/hip_bf16.h>
 #include <stdint.h>
@@ -44,148 +62,17 @@ namespace simd2 {
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// 1. top-k of a materialised distance matrix (reference layout)
-// ---------------------------------------------------------------------------
-constexpr int SEL_KMAX = 32;
-constexpr int SEL_Q = 64;      // queries per workgroup (one per lane)
-constexpr int SEL_U = 16;      // distance loads in flight per lane
-// Waves splitting the references: as many as the merge lists fit in LDS
-// (PARTS x KMAX x 64 x 8 bytes).
-template <int KMAX> constexpr int sel_parts() { return KMAX <= 16 ? 16 : 8; }
-
-// The reference's top-k (apps/pld/cuASR/knncuda.cu:170-237) keeps a sorted
-// list: while it holds fewer than k entries every element enters; then an
-// element enters only with a distance strictly below the k-th, equal
-// distances stay in index order, and the last entry drops out.  Here the list
-// is kept unsorted with its largest entry cached: an element enters under the
-// same rule and replaces the entry that is largest by (distance, index) --
-// the one the sorted list drops -- and the k entries are sorted by (distance,
-// index) at the end, which gives the same list.  An insertion rescans the k
-// entries for the new maximum instead of shifting them.
-template <int KMAX>
-__device__ __forceinline__ void max_insert(float (&ld)[KMAX], int (&li)[KMAX], int &n,
-                                           float &thr, int &tpos, int k, float d, int idx) {
-  if (n < k) {
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j)
-      if (j == n) {
-        ld[j] = d;
-        li[j] = idx;
-      }
-    if (++n < k) return;
-  } else {
-    if (!(d < thr)) return;
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j)
-      if (j == tpos) {
-        ld[j] = d;
-        li[j] = idx;
-      }
-  }
-  float md = ld[0];
-  int mi = li[0], mp = 0;
-#pragma unroll
-  for (int j = 1; j < KMAX; ++j)
-    if (j < k && (ld[j] > md || (ld[j] == md && li[j] > mi))) {
-      md = ld[j];
-      mi = li[j];
-      mp = j;
-    }
-  thr = md;
-  tpos = mp;
-}
-
-template <int KMAX, int SEL_PARTS = sel_parts<KMAX>()>
-__global__ __launch_bounds__(SEL_Q * SEL_PARTS) void knn_select_kernel(
-    const float *__restrict__ dist, int Q, int R, int k, float *out_d, int *out_i, int do_sqrt) {
-  __shared__ float sd[SEL_PARTS][KMAX][SEL_Q];
-  __shared__ int si[SEL_PARTS][KMAX][SEL_Q];
-  __shared__ int sn[SEL_PARTS][SEL_Q];
-  const int lane = tid_x() % 64;
-  const int part = tid_x() / 64;
-  const int q = bid_x() * SEL_Q + lane;
-  const int chunk = (R + SEL_PARTS - 1) / SEL_PARTS;
-  const int r0 = part * chunk;
-  const int r1 = min(R, r0 + chunk);
-  float ld[KMAX];
-  int li[KMAX];
-#pragma unroll
-  for (int j = 0; j < KMAX; ++j) {  // unfilled entries sort last
-    ld[j] = __builtin_inff();
-    li[j] = 0x7fffffff;
-  }
-  int n = 0, tpos = 0;
-  float thr = __builtin_inff();
-  if (q < Q) {
-    int r = r0;
-    for (; r + SEL_U <= r1; r += SEL_U) {
-      float v[SEL_U];
-#pragma unroll
-      for (int u = 0; u < SEL_U; ++u) v[u] = dist[(int64_t)(r + u) * Q + q];
-#pragma unroll
-      for (int u = 0; u < SEL_U; ++u) max_insert<KMAX>(ld, li, n, thr, tpos, k, v[u], r + u);
-    }
-    for (; r < r1; ++r) max_insert<KMAX>(ld, li, n, thr, tpos, k, dist[(int64_t)r * Q + q], r);
-  }
-  // sort by (distance, index): odd-even transposition over the registers
-#pragma unroll
-  for (int pass = 0; pass < KMAX; ++pass)
-#pragma unroll
-    for (int j = pass & 1; j + 1 < KMAX; j += 2) {
-      const bool sw = ld[j + 1] < ld[j] || (ld[j + 1] == ld[j] && li[j + 1] < li[j]);
-      const float td = sw ? ld[j + 1] : ld[j];
-      const int ti = sw ? li[j + 1] : li[j];
-      ld[j + 1] = sw ? ld[j] : ld[j + 1];
-      li[j + 1] = sw ? li[j] : li[j + 1];
-      ld[j] = td;
-      li[j] = ti;
-    }
-#pragma unroll
-  for (int j = 0; j < KMAX; ++j) {
-    sd[part][j][lane] = ld[j];
-    si[part][j][lane] = li[j];
-  }
-  sn[part][lane] = n;
-  __syncthreads();
-  if (part != 0 || q >= Q) return;
-  // Merge of the parts' sorted lists: part p holds lower indices than part
-  // p + 1, so on equal distances the lowest part wins, which keeps index
-  // order.
-  int pos[SEL_PARTS];
-#pragma unroll
-  for (int p = 0; p < SEL_PARTS; ++p) pos[p] = 0;
-  for (int j = 0; j < k; ++j) {
-    int best = -1;
-    float bd = 0.f;
-#pragma unroll
-    for (int p = 0; p < SEL_PARTS; ++p) {
-      if (pos[p] < sn[p][lane]) {
-        const float dp = sd[p][pos[p]][lane];
-        if (best < 0 || dp < bd) {
-          best = p;
-          bd = dp;
-        }
-      }
-    }
-    if (best < 0) break;
-    int bi = 0;
-#pragma unroll
-    for (int p = 0; p < SEL_PARTS; ++p)
-      if (p == best) {
-        bi = si[p][pos[p]][lane];
-        ++pos[p];
-      }
-    out_d[(int64_t)j * Q + q] = do_sqrt ? sqrtf(bd) : bd;
-    out_i[(int64_t)j * Q + q] = bi;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// 2. precision form
-// ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int QG = 2;        // groups of 16 queries per wave
+constexpr int CQ = 64 * QG;  // queries per workgroup (4 waves)
+constexpr int CR = 64;       // references per tile (tile minima granularity)
+constexpr int CK_MARGIN = 16;
+constexpr int KMAX = 256;    // largest k of the precision form
+constexpr int CAND_CAP = 1024;
+constexpr int FB_GRID = 64;  // workgroups of the overflow pass
+constexpr int MEAN_ROWS = 64;
 
 // Minimum over the 16 lanes of a DPP row (lanes 16j .. 16j+15), in every lane
 // of the row: rotations by 8, 4, 2 and 1 within the row, each one VALU min
@@ -198,18 +85,46 @@ __device__ __forceinline__ float row_min16(float m) {
   return m;
 }
 
-// One wave per point: the FP32 coordinates split into a BF16 high part and a
-// BF16 low part (x - hi), dims zero-padded to dpad, stored [point][hi | lo],
-// and the FP32 squared norm (wave-summed).
+// --- 1. centre --------------------------------------------------------------
+// partial[b][j] = sum of ref[r][j] over rows r of block b (double, row order).
+__global__ __launch_bounds__(256) void mean_partial_kernel(const float *__restrict__ P,
+                                                           int64_t ldp, int n, int d,
+                                                           double *partial) {
+  const int r0 = bid_x() * MEAN_ROWS, r1 = min(n, r0 + MEAN_ROWS);
+  for (int j = tid_x(); j < d; j += 256) {
+    double s = 0.0;
+    for (int r = r0; r < r1; ++r) s += (double)P[(int64_t)r * ldp + j];
+    partial[(int64_t)bid_x() * d + j] = s;
+  }
+}
+
+// mu[j] = (sum of the partials in block order) / n; clears the norm maximum.
+__global__ __launch_bounds__(256) void mean_final_kernel(const double *__restrict__ partial,
+                                                         int nblk, int d, int n, float *mu,
+                                                         unsigned *rnmax) {
+  for (int j = bid_x() * 256 + tid_x(); j < d; j += gridDim.x * 256) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * d + j];
+    mu[j] = (float)(s / n);
+  }
+  if (bid_x() == 0 && tid_x() == 0) *rnmax = 0u;
+}
+
+// --- 2. prep ----------------------------------------------------------------
+// One wave per point: x - mu split into BF16 high and low parts (dims
+// zero-padded to dpad, stored [point][hi | lo]), the FP32 squared norm of
+// x - mu (wave-summed), and for the references the largest norm (its bits:
+// norms are >= 0, so integer order is float order).
 __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, int64_t ldp,
-                                                   int n, int d, int dpad, __bf16 *Pb,
-                                                   float *norm) {
+                                                   int n, int d, int dpad,
+                                                   const float *__restrict__ mu, __bf16 *Pb,
+                                                   float *norm, unsigned *nmax) {
   const int lane = tid_x() % 64;
   const int p = bid_x() * 4 + tid_x() / 64;
   if (p >= n) return;
   float s = 0.f;
   for (int j = lane; j < dpad; j += 64) {
-    const float x = j < d ? P[(int64_t)p * ldp + j] : 0.f;
+    const float x = j < d ? P[(int64_t)p * ldp + j] - mu[j] : 0.f;
     s += x * x;
     const __bf16 hi = __float2bfloat16(x);
     Pb[(int64_t)p * 2 * dpad + j] = hi;
@@ -217,54 +132,162 @@ __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, 
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane == 0) norm[p] = s;
+  if (lane == 0) {
+    norm[p] = s;
+    if (nmax) atomicMax(nmax, __float_as_uint(s));
+  }
 }
 
-constexpr int QG = 2;       // groups of 16 queries per wave
-constexpr int CQ = 64 * QG; // queries per workgroup (4 waves)
-constexpr int CR = 64;      // references per tile
-constexpr int CAND_K = 32;  // tau bounds the CAND_K-th best distance
-// Candidate buffer per query: with no ties at tau at most CAND_K tiles hold
-// references at or under it, 64 each; a pathological tie of more tile
-// minima at exactly tau keeps the first CAND_CAP arrivals.
-constexpr int CAND_CAP = CAND_K * CR;
+// --- 3 / 5. the distance passes ----------------------------------------------
+struct PassArgs {
+  const float *qn, *rn, *thr;
+  float *tmin;
+  int ntiles;
+  float *cand_d;
+  int *cand_i, *cand_n, *ovf_n, *ovf_list;
+};
 
-// q.r ~ qh.rh + qh.rl + ql.rh: three BF16 MFMAs per k-step carry ~16
-// significant bits, which the norm expansion needs (centres ~N(0, 10^2) make
-// ||q||^2 + ||r||^2 ~ 100x the neighbour distances); a single BF16 product
-// leaves errors of ~10 on distances ~250 and loses true neighbours.
-// PASS 1: tmin[q][tile] = min over the tile; PASS 2: append r with d <= tau[q].
+// LDS candidate slots of a workgroup's queries (pass 2; one dummy slot in pass 1).
+template <int PASS, int SLOTS>
+struct CandStage {
+  static constexpr int CB = PASS == 2 ? SLOTS : 1;
+  float d[CQ][CB];
+  int i[CQ][CB];
+  int n[CQ];
+};
+
+// One candidate straight to the query's global list; the append that
+// overflows the list puts the query on the overflow list (once).
+__device__ __forceinline__ void append_global(const PassArgs &a, int q, float dd, int r) {
+  const int at = atomicAdd(&a.cand_n[q], 1);
+  if (at < CAND_CAP) {
+    a.cand_d[(int64_t)q * CAND_CAP + at] = dd;
+    a.cand_i[(int64_t)q * CAND_CAP + at] = r;
+  } else if (at == CAND_CAP) {
+    a.ovf_list[atomicAdd(a.ovf_n, 1)] = q;
+  }
+}
+
+// The epilogue of one workgroup step over RB blocks of 16 references
+// (r0 .. r0 + 16 RB): acc[g][rb] holds the dot products of this lane's 4
+// queries of group g with reference r0 + rb*16 + lane%16.
+template <int PASS, int RB, typename Stage>
+__device__ __forceinline__ void pass_epilogue(const f32x4 (&acc)[QG][RB], const float (&qnv)[QG][4],
+                                              const float (&tv)[QG][4], const int (&qv)[QG][4],
+                                              int q0, int Q, int r0, int rb1, const float *Rn,
+                                              Stage &cs, const PassArgs &a, int lane) {
+  float rnv[RB];
+  bool valid[RB];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    rnv[rb] = Rn[rb * 16 + lane % 16];
+    valid[rb] = r0 + rb * 16 + lane % 16 < rb1;
+  }
+  if constexpr (PASS == 1) {
+#pragma unroll
+    for (int g = 0; g < QG; ++g)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int t = 0; t < RB / 4; ++t) {  // each 64-reference tile of the step
+          float m = __builtin_inff();
+#pragma unroll
+          for (int rb = 4 * t; rb < 4 * t + 4; ++rb)
+            if (valid[rb]) m = fminf(m, qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v]);
+          // the 16 lanes of this query (one DPP row) hold the tile's 64 references
+          m = row_min16(m);
+          const int tile = r0 / CR + t;
+          if (lane % 16 == 0 && qv[g][v] < Q && r0 + CR * t < rb1)
+            a.tmin[(int64_t)qv[g][v] * a.ntiles + tile] = m;
+        }
+  } else {
+#pragma unroll
+    for (int g = 0; g < QG; ++g)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+          const float dd = qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v];
+          if (valid[rb] && dd <= tv[g][v]) {
+            const int q = qv[g][v], ql = q - q0;
+            const int at = atomicAdd(&cs.n[ql], 1);
+            const int r = r0 + rb * 16 + lane % 16;
+            if (at < Stage::CB) {
+              cs.d[ql][at] = dd;
+              cs.i[ql][at] = r;
+            } else {
+              append_global(a, q, dd, r);
+            }
+          }
+        }
+  }
+}
+
+// The LDS candidate slots of a workgroup to the global lists.
+template <int PASS, typename Stage>
+__device__ __forceinline__ void flush_candidates(Stage &cs, const PassArgs &a, int q0, int Q,
+                                                 int tid) {
+  if constexpr (PASS == 2) {
+    __syncthreads();
+    for (int ql = tid; ql < CQ; ql += 256) {
+      const int q = q0 + ql, nb = min(cs.n[ql], Stage::CB);
+      if (q < Q && nb > 0) {
+        const int b = atomicAdd(&a.cand_n[q], nb);
+        for (int e = 0; e < nb; ++e) {
+          if (b + e < CAND_CAP) {
+            a.cand_d[(int64_t)q * CAND_CAP + b + e] = cs.d[ql][e];
+            a.cand_i[(int64_t)q * CAND_CAP + b + e] = cs.i[ql][e];
+          } else if (b + e == CAND_CAP) {
+            a.ovf_list[atomicAdd(a.ovf_n, 1)] = q;
+          }
+        }
+      }
+    }
+  }
+}
+
+// The 4 queries per group whose accumulator rows this lane holds: norms and thresholds.
+template <int PASS>
+__device__ __forceinline__ void query_rows(const PassArgs &a, int q0, int Q, int wave, int lane,
+                                           float (&qnv)[QG][4], float (&tv)[QG][4],
+                                           int (&qv)[QG][4]) {
+#pragma unroll
+  for (int g = 0; g < QG; ++g)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qv[g][v] = q0 + wave * 16 * QG + g * 16 + 4 * (lane / 16) + v;
+      qnv[g][v] = qv[g][v] < Q ? a.qn[qv[g][v]] : 0.f;
+      tv[g][v] = PASS == 2 && qv[g][v] < Q ? a.thr[qv[g][v]] : -__builtin_inff();
+    }
+}
+
+// Up to 160 dims: the workgroup's 128 queries' A fragments stay in registers
+// (every reference fragment read from LDS feeds QG x 3 MFMAs); references are
+// streamed through LDS in 64-row tiles, the next tile's loads in flight while
+// the current one is multiplied.  Workgroup b takes query block b / slices and
+// reference slice b % slices (with a multiple of 8 slices, workgroups run on
+// XCD b % 8, so each XCD's L2 keeps its slices).
 template <int DPAD, int PASS>
-__global__ __launch_bounds__(256) void knn_pass_kernel(
-    const __bf16 *__restrict__ Qb, const __bf16 *__restrict__ Rb, const float *__restrict__ qn,
-    const float *__restrict__ rn, int Q, int R, int slices, int slice, float *tmin, int ntiles,
-    const float *__restrict__ tau, float *cand_d, int *cand_i, int *cand_n) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void knn_pass_kernel(
+    const __bf16 *__restrict__ Qb, const __bf16 *__restrict__ Rb, int Q, int R, int slices,
+    int slice, PassArgs a) {
   constexpr int KS = DPAD / 32;             // MFMA k-steps
   constexpr int RLD = 2 * DPAD + 8;         // LDS row: hi | lo, +16 bytes of padding
   constexpr int CHUNKS = CR * 2 * DPAD / 8; // 16-byte chunks per reference tile
   constexpr int PER = CHUNKS / 256;         // chunks per thread
   __shared__ __attribute__((aligned(16))) __bf16 Rt[CR][RLD];
   __shared__ float Rn[CR];
-  constexpr int CB = PASS == 2 ? 16 : 1;  // LDS candidate slots per query
-  __shared__ float Cd[CQ][CB];
-  __shared__ int Ci[CQ][CB];
-  __shared__ int Cn[CQ];
+  __shared__ CandStage<PASS, 16> cs;
 
   const int tid = tid_x();
   const int lane = tid % 64;
   const int wave = tid / 64;
   if (PASS == 2)
-    for (int i = tid; i < CQ; i += 256) Cn[i] = 0;  // published by the loop's first barrier
-  // Workgroup b takes query block b / slices and reference slice b % slices:
-  // with 8 slices, workgroups run on XCD b % 8, so each XCD streams one slice
-  // and its L2 keeps it.
+    for (int i = tid; i < CQ; i += 256) cs.n[i] = 0;  // published by the loop's first barrier
   const int q0 = (bid_x() / slices) * CQ;
   const int rb0 = (bid_x() % slices) * slice;
   const int rb1 = min(R, rb0 + slice);
 
-  // A fragments of this wave's QG groups of 16 queries (every reference
-  // fragment read from LDS feeds QG x 3 MFMAs): lane l holds query l%16 of
-  // each group, dims 8*(l/16) .. +7 of every 32-wide k-step, high and low.
   bf16x8 ah[QG][KS], al[QG][KS];
 #pragma unroll
   for (int g = 0; g < QG; ++g) {
@@ -284,20 +307,10 @@ __global__ __launch_bounds__(256) void knn_pass_kernel(
       }
     }
   }
-  // The 4 queries per group whose accumulator rows this lane holds: norms and tau.
   float qnv[QG][4], tv[QG][4];
   int qv[QG][4];
-#pragma unroll
-  for (int g = 0; g < QG; ++g)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      qv[g][v] = q0 + wave * 16 * QG + g * 16 + 4 * (lane / 16) + v;
-      qnv[g][v] = qv[g][v] < Q ? qn[qv[g][v]] : 0.f;
-      tv[g][v] = PASS == 2 && qv[g][v] < Q ? tau[qv[g][v]] : -__builtin_inff();
-    }
+  query_rows<PASS>(a, q0, Q, wave, lane, qnv, tv, qv);
 
-  // Register prefetch of the next reference tile (global -> registers while
-  // the current tile is multiplied, registers -> LDS after).
   uint4 pre[PER];
   float pren = 0.f;
   auto fetch = [&](int r0) {
@@ -310,7 +323,7 @@ __global__ __launch_bounds__(256) void knn_pass_kernel(
       pre[h] = make_uint4(0u, 0u, 0u, 0u);
       if (r < rb1) pre[h] = *reinterpret_cast<const uint4 *>(Rb + (int64_t)r * 2 * DPAD + cc);
     }
-    if (tid < CR) pren = (r0 + tid < rb1) ? rn[r0 + tid] : 0.f;
+    if (tid < CR) pren = (r0 + tid < rb1) ? a.rn[r0 + tid] : 0.f;
   };
   auto stage = [&]() {
 #pragma unroll
@@ -349,76 +362,112 @@ __global__ __launch_bounds__(256) void knn_pass_kernel(
         }
       }
     }
-    // Distances of this lane's queries x references rb*16 + lane%16;
-    // references past the slice end take no part.
-    float rnv[4];
-    bool valid[4];
+    pass_epilogue<PASS, 4>(acc, qnv, tv, qv, q0, Q, r0, rb1, Rn, cs, a, lane);
+  }
+  flush_candidates<PASS>(cs, a, q0, Q, tid);
+}
+
+// Any number of dims: queries and references both streamed through LDS in
+// 32-dim slabs (double-buffered, the next slab's loads in flight), 128
+// queries x 128 references per workgroup step (each wave 32 x 128).
+constexpr int SRB = 8;          // 16-reference blocks per step
+constexpr int SREF = 16 * SRB;  // 128 references per step
+constexpr int SLD = 72;         // LDS row: 32 hi + 32 lo bf16, +16 bytes of padding
+
+template <int PASS>
+__global__ __launch_bounds__(256) void knn_pass_stream_kernel(
+    const __bf16 *__restrict__ Qb, const __bf16 *__restrict__ Rb, int Q, int R, int dpad,
+    int slices, int slice, PassArgs a) {
+  __shared__ __attribute__((aligned(16))) __bf16 Qs[2][CQ][SLD];
+  __shared__ __attribute__((aligned(16))) __bf16 Rs[2][SREF][SLD];
+  __shared__ float Rn[SREF];
+  // 4 slots per query keep two workgroups' LDS within the CU's 160 KB
+  __shared__ CandStage<PASS, 4> cs;
+
+  const int tid = tid_x();
+  const int lane = tid % 64;
+  const int wave = tid / 64;
+  if (PASS == 2)
+    for (int i = tid; i < CQ; i += 256) cs.n[i] = 0;
+  const int q0 = (bid_x() / slices) * CQ;
+  const int rb0 = (bid_x() % slices) * slice;
+  const int rb1 = min(R, rb0 + slice);
+  float qnv[QG][4], tv[QG][4];
+  int qv[QG][4];
+  query_rows<PASS>(a, q0, Q, wave, lane, qnv, tv, qv);
+  const int ksteps = dpad / 32;
+
+  // Slab ks of the query block and of the reference step at r0: each of the
+  // 128 + 128 rows is 64 B of hi and 64 B of lo; thread t moves half t % 2
+  // (hi, lo) of query row t / 2 and of reference row t / 2, 4 x 16 B each.
+  uint4 pq[4], pr[4];
+  auto fetch = [&](int r0, int ks) {
+    const int row = tid / 2, half = tid % 2;
+    const int q = q0 + row, r = r0 + row;
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
-      rnv[rb] = Rn[rb * 16 + lane % 16];
-      valid[rb] = r0 + rb * 16 + lane % 16 < rb1;
+    for (int c = 0; c < 4; ++c) {
+      pq[c] = make_uint4(0u, 0u, 0u, 0u);
+      pr[c] = make_uint4(0u, 0u, 0u, 0u);
+      if (q < Q)
+        pq[c] = *reinterpret_cast<const uint4 *>(Qb + (int64_t)q * 2 * dpad + half * dpad +
+                                                 ks * 32 + 8 * c);
+      if (r < rb1)
+        pr[c] = *reinterpret_cast<const uint4 *>(Rb + (int64_t)r * 2 * dpad + half * dpad +
+                                                 ks * 32 + 8 * c);
     }
+  };
+  auto stage = [&](int buf) {
+    const int row = tid / 2, half = tid % 2;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      *reinterpret_cast<uint4 *>(&Qs[buf][row][half * 32 + 8 * c]) = pq[c];
+      *reinterpret_cast<uint4 *>(&Rs[buf][row][half * 32 + 8 * c]) = pr[c];
+    }
+  };
+
+  for (int r0 = rb0; r0 < rb1; r0 += SREF) {
+    __syncthreads();  // the previous step's epilogue has read Rn and the slabs
+    if (tid < SREF) Rn[tid] = (r0 + tid < rb1) ? a.rn[r0 + tid] : 0.f;
+    f32x4 acc[QG][SRB];
 #pragma unroll
     for (int g = 0; g < QG; ++g)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        if constexpr (PASS == 1) {  // the tile minimum of each query
-          float m = __builtin_inff();
-#pragma unroll
-          for (int rb = 0; rb < 4; ++rb)
-            if (valid[rb]) m = fminf(m, qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v]);
-          // the 16 lanes of this query (same lane / 16, one DPP row) hold the
-          // tile's 64 references
-          m = row_min16(m);
-          if (lane % 16 == 0 && qv[g][v] < Q) tmin[(int64_t)qv[g][v] * ntiles + r0 / CR] = m;
-        }
-      }
-    if constexpr (PASS == 2) {
-      // References at or under tau (about CAND_K per query over all slices)
-      // take a slot of the query's LDS buffer; overflow goes straight to the
-      // global buffer.
-#pragma unroll
-      for (int g = 0; g < QG; ++g)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-          for (int rb = 0; rb < 4; ++rb) {
-            const float dd = qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v];
-            if (valid[rb] && dd <= tv[g][v]) {
-              const int q = qv[g][v], ql = q - q0;
-              const int at = atomicAdd(&Cn[ql], 1);
-              const int r = r0 + rb * 16 + lane % 16;
-              if (at < CB) {
-                Cd[ql][at] = dd;
-                Ci[ql][at] = r;
-              } else {
-                const int gat = atomicAdd(&cand_n[q], 1);
-                if (gat < CAND_CAP) {
-                  cand_d[(int64_t)q * CAND_CAP + gat] = dd;
-                  cand_i[(int64_t)q * CAND_CAP + gat] = r;
-                }
-              }
-            }
-          }
-    }
-  }
-  if constexpr (PASS == 2) {
-    // The LDS buffers to the global ones: one atomic per query.
+      for (int rb = 0; rb < SRB; ++rb) acc[g][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    fetch(r0, 0);
+    stage(0);
     __syncthreads();
-    for (int ql = tid; ql < CQ; ql += 256) {
-      const int q = q0 + ql, nb = min(Cn[ql], CB);
-      if (q < Q && nb > 0) {
-        const int b = atomicAdd(&cand_n[q], nb);
-        for (int e = 0; e < nb && b + e < CAND_CAP; ++e) {
-          cand_d[(int64_t)q * CAND_CAP + b + e] = Cd[ql][e];
-          cand_i[(int64_t)q * CAND_CAP + b + e] = Ci[ql][e];
+    for (int ks = 0; ks < ksteps; ++ks) {
+      const int buf = ks & 1;
+      if (ks + 1 < ksteps) fetch(r0, ks + 1);
+      bf16x8 ah[QG], al[QG];
+#pragma unroll
+      for (int g = 0; g < QG; ++g) {
+        const __bf16 *row = &Qs[buf][wave * 16 * QG + g * 16 + lane % 16][8 * (lane / 16)];
+        ah[g] = *reinterpret_cast<const bf16x8 *>(row);
+        al[g] = *reinterpret_cast<const bf16x8 *>(row + 32);
+      }
+#pragma unroll
+      for (int rb = 0; rb < SRB; ++rb) {
+        const __bf16 *row = &Rs[buf][rb * 16 + lane % 16][8 * (lane / 16)];
+        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(row);
+        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(row + 32);
+#pragma unroll
+        for (int g = 0; g < QG; ++g) {
+          acc[g][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[g], bh, acc[g][rb], 0, 0, 0);
+          acc[g][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[g], bl, acc[g][rb], 0, 0, 0);
+          acc[g][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[g], bh, acc[g][rb], 0, 0, 0);
         }
       }
+      // the other buffer was last read in the previous step, before its barrier
+      if (ks + 1 < ksteps) stage(buf ^ 1);
+      __syncthreads();
     }
+    pass_epilogue<PASS, SRB>(acc, qnv, tv, qv, q0, Q, r0, rb1, Rn, cs, a, lane);
   }
+  flush_candidates<PASS>(cs, a, q0, Q, tid);
 }
 
-// Order-preserving 32-bit key of a (non-NaN) float.
+// --- 4. tau -------------------------------------------------------------------
 __device__ __forceinline__ uint32_t fkey(float f) {
   const uint32_t u = __float_as_uint(f);
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -427,31 +476,36 @@ __device__ __forceinline__ float fkey_inv(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-// tau[q] = the CAND_K-th smallest of the query's tile minima (+inf when it has
-// fewer tiles), by bisection over the keys; one wave per query.  Also clears
-// the query's candidate count.
+// thr[q] = (the ck-th smallest of the query's tile minima) + 2 E_q, +inf when
+// it has fewer than ck tiles; by bisection over the keys, one wave per query.
+// Also clears the query's candidate count and exact flag (and, in the first
+// wave, the counts of the overflow list and of the re-rank's LDS list).
 __global__ __launch_bounds__(256) void knn_tau_kernel(const float *__restrict__ tmin, int ntiles,
-                                                      int Q, float *tau, int *cand_n) {
+                                                      int Q, int ck, float e_coef,
+                                                      const float *__restrict__ qn,
+                                                      const unsigned *__restrict__ rnmax,
+                                                      float *thr, int *cand_n, int *cand_exact,
+                                                      int *ovf_n, int *big_n) {
   const int lane = tid_x() % 64;
   const int q = bid_x() * 4 + tid_x() / 64;
+  if (q == 0 && lane == 0) {
+    *ovf_n = 0;
+    *big_n = 0;
+  }
   if (q >= Q) return;
   const float *t = tmin + (int64_t)q * ntiles;
-  uint32_t lo = 0u, hi = 0xffffffffu;  // smallest key with >= CAND_K minima at or under it
-  constexpr int TP = 8;                 // keys held in registers per lane (ntiles <= 512)
+  uint32_t lo = 0u, hi = 0xffffffffu;  // smallest key with >= ck minima at or under it
+  constexpr int TP = 8;                 // keys held in registers per lane
   uint32_t kv[TP];
 #pragma unroll
   for (int i = 0; i < TP; ++i) {
     const int j = lane + 64 * i;
     kv[i] = j < ntiles ? fkey(t[j]) : 0xffffffffu;
   }
-  if (ntiles < CAND_K) {
-    lo = fkey(__builtin_inff());
-  } else {
+  float res = __builtin_inff();
+  if (ntiles >= ck) {
     while (lo < hi) {
       const uint32_t mid = lo + (hi - lo) / 2;
-      // The wave's count of keys <= mid: one ballot per register of keys,
-      // counted in scalar registers (no cross-lane shuffles); keys past the
-      // registers (ntiles > 512) are counted per lane and summed.
       int c = 0;
 #pragma unroll
       for (int i = 0; i < TP; ++i)
@@ -463,21 +517,105 @@ __global__ __launch_bounds__(256) void knn_tau_kernel(const float *__restrict__ 
         for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
         c += e;
       }
-      if (c >= CAND_K)
+      if (c >= ck)
         hi = mid;
       else
         lo = mid + 1;
     }
+    const float eq = e_coef * (qn[q] + __uint_as_float(*rnmax));
+    res = fkey_inv(lo) + 2.0f * eq;
+    res = res + fabsf(res) * 1e-6f;  // the sums above round by less than this
   }
   if (lane == 0) {
-    tau[q] = fkey_inv(lo);
+    thr[q] = res;
     cand_n[q] = 0;
+    cand_exact[q] = 0;
   }
 }
 
-constexpr int RR_M = 32;        // candidates re-ranked exactly per query
-constexpr float RR_TOL = 1e-5f; // relative FP32 gap treated as a near tie
+// --- canonical exact distances ---------------------------------------------------
+// The exact FP32 squared distance: eight partial sums s_p over dims j = p mod 8
+// in increasing j, combined as ((s0+s1)+(s2+s3))+((s4+s5)+(s6+s7)).  The wave
+// form (8 lanes per candidate, combined by xor shuffles 1, 2, 4) and this
+// thread form give the same bits.
+__device__ __forceinline__ float exact_d32_thread(const float *__restrict__ q,
+                                                  const float *__restrict__ r, int d) {
+  float s[8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) s[p] = 0.f;
+  int j = 0;
+  for (; j + 8 <= d; j += 8)
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const float t = q[j + p] - r[j + p];
+      s[p] = s[p] + t * t;
+    }
+#pragma unroll
+  for (int p = 0; p < 8; ++p)
+    if (j + p < d) {
+      const float t = q[j + p] - r[j + p];
+      s[p] = s[p] + t * t;
+    }
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
 
+// --- 6. overflow: exact distances to every reference ----------------------------
+// One workgroup per listed query (grid-stride): exact distances into the
+// workgroup's scratch row, then the min(R, CAND_CAP) smallest (distance,
+// index) keys (by bisection on the key) become the query's candidates,
+// marked exact.
+__global__ __launch_bounds__(256) void knn_overflow_kernel(
+    const float *__restrict__ Qf, int64_t ldq, const float *__restrict__ Rf, int64_t ldr, int R,
+    int d, const int *__restrict__ ovf_n, const int *__restrict__ ovf_list, float *scratch,
+    float *cand_d, int *cand_i, int *cand_n, int *cand_exact) {
+  __shared__ int cnt_s;
+  const int tid = tid_x();
+  const int n_ovf = *ovf_n;
+  float *row = scratch + (int64_t)bid_x() * R;
+  for (int b = bid_x(); b < n_ovf; b += FB_GRID) {
+    const int q = ovf_list[b];
+    for (int r = tid; r < R; r += 256)
+      row[r] = exact_d32_thread(Qf + (int64_t)q * ldq, Rf + (int64_t)r * ldr, d);
+    __threadfence_block();
+    __syncthreads();
+    const int want = min(R, CAND_CAP);
+    // the smallest 64-bit key (fkey(d) << 32 | r) with >= want keys at or under it
+    uint64_t lo = 0ull, hi = ~0ull;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (tid == 0) cnt_s = 0;
+      __syncthreads();
+      int c = 0;
+      for (int r = tid; r < R; r += 256)
+        c += (((uint64_t)fkey(row[r]) << 32) | (uint32_t)r) <= mid ? 1 : 0;
+      atomicAdd(&cnt_s, c);
+      __syncthreads();
+      const int total = cnt_s;
+      __syncthreads();
+      if (total >= want)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    if (tid == 0) cnt_s = 0;
+    __syncthreads();
+    for (int r = tid; r < R; r += 256) {
+      if ((((uint64_t)fkey(row[r]) << 32) | (uint32_t)r) <= lo) {
+        const int at = atomicAdd(&cnt_s, 1);  // exactly `want` keys pass
+        cand_d[(int64_t)q * CAND_CAP + at] = row[r];
+        cand_i[(int64_t)q * CAND_CAP + at] = r;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      cand_n[q] = cnt_s;
+      cand_exact[q] = 1;
+    }
+    __syncthreads();  // the scratch row is reused by the next listed query
+  }
+}
+
+// --- 7. re-rank -----------------------------------------------------------------
 __device__ __forceinline__ bool key_less(float d, int i, float e, int j) {
   return d < e || (d == e && i < j);
 }
@@ -485,233 +623,443 @@ __device__ __forceinline__ bool key_less64(double d, int i, double e, int j) {
   return d < e || (d == e && i < j);
 }
 
-__global__ __launch_bounds__(256) void rerank_kernel(const float *__restrict__ Qf, int64_t ldq,
-                                                     const float *__restrict__ Rf, int64_t ldr,
-                                                     int Q, int d, int k,
-                                                     const float *__restrict__ cand_d,
-                                                     const int *__restrict__ cand_i,
-                                                     const int *__restrict__ cand_n, int ncand,
-                                                     float *out_d, int *out_i, int do_sqrt,
-                                                     int *stats) {
+__device__ __forceinline__ double shfl_d(double v, int src) {
+  const uint64_t u = __double_as_longlong(v);
+  const int lo = __shfl((int)(uint32_t)u, src), hi = __shfl((int)(uint32_t)(u >> 32), src);
+  return __longlong_as_double(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Exact FP32 distances of candidates m0 .. m0 + 7 (lanes 8g .. 8g+7 sum dims
+// p, p + 8, ... of candidate m0 + g), combined by the xor tree; lane
+// 8g + p receives candidate m0 + g's distance.
+__device__ __forceinline__ float exact_d32_wave(const float *__restrict__ qrow,
+                                                const float *__restrict__ Rf, int64_t ldr, int d,
+                                                int r, bool live, int part) {
+  float s = 0.f;
+  if (live) {
+    const float *rrow = Rf + (int64_t)r * ldr;
+    for (int j = part; j < d; j += 8) {
+      const float t = qrow[j] - rrow[j];
+      s = s + t * t;
+    }
+  }
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
+  s += __shfl_xor(s, 4);
+  return s;
+}
+
+// FP64 squared distance of reference r, summed by the whole wave.
+__device__ __forceinline__ double exact_d64_wave(const float *__restrict__ qrow,
+                                                 const float *__restrict__ Rf, int64_t ldr, int d,
+                                                 int r, int lane) {
+  const float *rrow = Rf + (int64_t)r * ldr;
+  double s = 0.0;
+  for (int j = lane; j < d; j += 64) {
+    const double t = (double)qrow[j] - (double)rrow[j];
+    s += t * t;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  return s;
+}
+
+// One wave per query with up to 64 candidates (lane m holds candidate m);
+// queries with more go to the list of rerank_big_kernel.
+__global__ __launch_bounds__(256) void rerank_small_kernel(
+    const float *__restrict__ Qf, int64_t ldq, const float *__restrict__ Rf, int64_t ldr, int Q,
+    int d, int k, float tol, const float *__restrict__ cand_d, const int *__restrict__ cand_i,
+    const int *__restrict__ cand_n, const int *__restrict__ cand_exact, float *out_d,
+    int *out_i, int do_sqrt, int *stats, int *big_n, int *big_list) {
   const int lane = tid_x() % 64;
   const int q = bid_x() * 4 + tid_x() / 64;
   if (q >= Q) return;
-  const float *cd = cand_d + (int64_t)q * ncand;
-  const int *ci = cand_i + (int64_t)q * ncand;
-  const int nq = min(cand_n[q], ncand);  // candidates of this query
-
-  // Step 1: the RR_M best candidates by (approximate distance, index).
-  int sel = -1;  // lane m < RR_M receives the m-th choice
-  if (nq <= 64) {
-    // one candidate per lane, bitonic sort of the wave by the key
-    float bd = lane < nq ? cd[lane] : __builtin_inff();
-    int bi = lane < nq ? ci[lane] : 0x7fffffff;
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1)
-#pragma unroll
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const float od = __shfl_xor(bd, j);
-        const int oi = __shfl_xor(bi, j);
-        // the lower lane of the pair keeps the smaller key in an ascending run
-        const bool up = (lane & k) == 0, low = (lane & j) == 0;
-        const bool take = (low == up) ? key_less(od, oi, bd, bi) : key_less(bd, bi, od, oi);
-        if (take) {
-          bd = od;
-          bi = oi;
-        }
-      }
-    if (lane < RR_M && bi != 0x7fffffff) sel = bi;
+  const int c = min(cand_n[q], CAND_CAP);
+  if (c > 64) {
+    if (lane == 0) big_list[atomicAdd(big_n, 1)] = q;
+    return;
   }
-  float last_d = -__builtin_inff();
-  int last_i = -1;
-  for (int m = 0; nq > 64 && m < RR_M; ++m) {
-    float bd = __builtin_inff();
-    int bi = 0x7fffffff;
-    for (int c = lane; c < nq; c += 64) {
-      const float dd = cd[c];
-      const int ii = ci[c];
-      if (ii >= 0 && key_less(last_d, last_i, dd, ii) && key_less(dd, ii, bd, bi)) {
-        bd = dd;
-        bi = ii;
-      }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const float od = __shfl_xor(bd, off);
-      const int oi = __shfl_xor(bi, off);
-      if (key_less(od, oi, bd, bi)) {
-        bd = od;
-        bi = oi;
-      }
-    }
-    if (bi == 0x7fffffff) break;  // fewer candidates than RR_M
-    if (lane == m) sel = bi;
-    last_d = bd;
-    last_i = bi;
-  }
-
-  // Step 2: exact FP32 distances, 8 candidates at a time: lanes 8c .. 8c+7
-  // sum dims part, part + 8, ... of candidate m0 + c, so that 8 row loads
-  // are in flight instead of one.
-  float d32 = __builtin_inff();
-  {
+  const bool exact = cand_exact[q] != 0;
+  const float *qrow = Qf + (int64_t)q * ldq;
+  int my_i = lane < c ? cand_i[(int64_t)q * CAND_CAP + lane] : 0x7fffffff;
+  float my_d = __builtin_inff();
+  if (exact) {
+    if (lane < c) my_d = cand_d[(int64_t)q * CAND_CAP + lane];
+  } else {
     const int grp = lane / 8, part = lane % 8;
-    for (int m0 = 0; m0 < RR_M; m0 += 8) {
-      const int r = __shfl(sel, m0 + grp);
-      float s = 0.f;
-      if (r >= 0)
-        for (int j = part; j < d; j += 8) {
-          const float t = Qf[(int64_t)q * ldq + j] - Rf[(int64_t)r * ldr + j];
-          s += t * t;
+    for (int m0 = 0; m0 < c; m0 += 8) {
+      const int r = __shfl(my_i, min(m0 + grp, 63));
+      const float s = exact_d32_wave(qrow, Rf, ldr, d, r, m0 + grp < c, part);
+      const float got = __shfl(s, 8 * ((lane - m0) & 7));
+      if (lane >= m0 && lane < m0 + 8 && lane < c) my_d = got;
+    }
+  }
+  // bitonic sort of the wave by (distance, index); padding sorts last
+#pragma unroll
+  for (int s = 2; s <= 64; s <<= 1)
+#pragma unroll
+    for (int j = s >> 1; j > 0; j >>= 1) {
+      const float od = __shfl_xor(my_d, j);
+      const int oi = __shfl_xor(my_i, j);
+      const bool up = (lane & s) == 0, low = (lane & j) == 0;
+      const bool take = (low == up) ? key_less(od, oi, my_d, my_i) : key_less(my_d, my_i, od, oi);
+      if (take) {
+        my_d = od;
+        my_i = oi;
+      }
+    }
+  // Near ties: bit i of T says candidate i lies within tol of candidate i - 1.
+  const float prev = __shfl(my_d, lane > 0 ? lane - 1 : 0);
+  const uint64_t T = __ballot(lane > 0 && lane < c && my_d - prev <= tol * my_d);
+  // this lane's run [start, end): start = the last position <= lane whose bit
+  // is clear, end = the first position > lane whose bit is clear (or c)
+  const uint64_t clear = ~T;
+  const uint64_t below = clear & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+  const int start = 63 - __clzll(below | 1ull);
+  const uint64_t above = (lane == 63) ? 0ull : (clear & ~((2ull << lane) - 1));
+  const int end = min(c, above ? __ffsll((long long)above) - 1 : 64);
+  const bool flag = lane < c && end - start >= 2 && start <= k;
+  const uint64_t F = __ballot(flag);
+  double my_x = (double)my_d;
+  if (F) {
+    for (uint64_t f = F; f; f &= f - 1) {
+      const int m = __ffsll((long long)f) - 1;
+      const double x = exact_d64_wave(qrow, Rf, ldr, d, __shfl(my_i, m), lane);
+      if (lane == m) my_x = x;
+    }
+    // re-sort by (FP64 distance where computed, else the FP32 one; index):
+    // runs are separated by gaps above tol that FP32's error cannot cross,
+    // so only the order inside the flagged runs can change
+#pragma unroll
+    for (int s = 2; s <= 64; s <<= 1)
+#pragma unroll
+      for (int j = s >> 1; j > 0; j >>= 1) {
+        const double ox = shfl_d(my_x, lane ^ j);
+        const float od = __shfl_xor(my_d, j);
+        const int oi = __shfl_xor(my_i, j);
+        const bool up = (lane & s) == 0, low = (lane & j) == 0;
+        const bool take = (low == up) ? key_less64(ox, oi, my_x, my_i)
+                                      : key_less64(my_x, my_i, ox, oi);
+        if (take) {
+          my_x = ox;
+          my_d = od;
+          my_i = oi;
         }
+      }
+    if (stats && lane == 0) {
+      atomicAdd(&stats[0], 1);
+      atomicAdd(&stats[1], (int)__popcll(F));
+    }
+  }
+  if (lane < k && lane < c) {
+    // a flagged candidate reports its FP64 distance rounded to FP32; for the
+    // others my_x is their FP32 distance exactly
+    const float outd = (float)my_x;
+    out_d[(int64_t)q * k + lane] = do_sqrt ? sqrtf(outd) : outd;
+    out_i[(int64_t)q * k + lane] = my_i;
+  }
+  if (stats && lane == 0 && exact) atomicAdd(&stats[2], 1);
+}
+
+// Queries with more than 64 candidates: one wave (one workgroup) per listed
+// query, grid-stride over the list, the candidates sorted in LDS.
+constexpr int BIG_GRID_RR = 512;
+__global__ __launch_bounds__(64) void rerank_big_kernel(const float *__restrict__ Qf, int64_t ldq,
+                                                        const float *__restrict__ Rf, int64_t ldr,
+                                                        int d, int k, float tol,
+                                                        const float *__restrict__ cand_d,
+                                                        const int *__restrict__ cand_i,
+                                                        const int *__restrict__ cand_n,
+                                                        const int *__restrict__ cand_exact,
+                                                        float *out_d, int *out_i, int do_sqrt,
+                                                        int *stats, const int *big_n,
+                                                        const int *big_list) {
+  __shared__ float sd[CAND_CAP];
+  __shared__ int si[CAND_CAP];
+  __shared__ double sx[CAND_CAP];
+  __shared__ int sflag[CAND_CAP];
+  __shared__ int nflag_s;
+  const int lane = tid_x();
+  const int nbig = *big_n;
+  for (int b = bid_x(); b < nbig; b += BIG_GRID_RR) {
+  const int q = big_list[b];
+  const int c = min(cand_n[q], CAND_CAP);
+  const bool exact = cand_exact[q] != 0;
+  const float *qrow = Qf + (int64_t)q * ldq;
+  const float *cd = cand_d + (int64_t)q * CAND_CAP;
+  const int *ci = cand_i + (int64_t)q * CAND_CAP;
+
+  // Exact FP32 distances: lanes 8g .. 8g+7 sum dims p, p + 8, ... of
+  // candidate m0 + g (8 candidates in flight), combined by the xor tree.
+  if (exact) {
+    for (int m = lane; m < c; m += 64) {
+      sd[m] = cd[m];
+      si[m] = ci[m];
+    }
+  } else {
+    const int grp = lane / 8, part = lane % 8;
+    for (int m0 = 0; m0 < c; m0 += 8) {
+      const int m = m0 + grp;
+      const int r = m < c ? ci[m] : 0;
+      float s = 0.f;
+      if (m < c) {
+        const float *rrow = Rf + (int64_t)r * ldr;
+        for (int j = part; j < d; j += 8) {
+          const float t = qrow[j] - rrow[j];
+          s = s + t * t;
+        }
+      }
       s += __shfl_xor(s, 1);
       s += __shfl_xor(s, 2);
       s += __shfl_xor(s, 4);
-      const float got = __shfl(s, 8 * ((lane - m0) & 7));
-      if (lane >= m0 && lane < m0 + 8 && sel >= 0) d32 = got;
+      if (part == 0 && m < c) {
+        sd[m] = s;
+        si[m] = r;
+      }
     }
   }
-  const int my_i = sel >= 0 ? sel : 0x7fffffff;
-
-  // Step 3: rank by (FP32 distance, index).
-  int rank = 0;
-  for (int m = 0; m < RR_M; ++m) {
-    const float od = __shfl(d32, m);
-    const int oi = __shfl(my_i, m);
-    if (m != lane && key_less(od, oi, d32, my_i)) ++rank;
+  int P = 1;
+  while (P < c) P <<= 1;
+  for (int m = c + lane; m < P; m += 64) {
+    sd[m] = __builtin_inff();
+    si[m] = 0x7fffffff;
   }
-  // Step 4: near ties that touch the top k get an FP64 distance.
-  bool flag = false;
-  for (int m = 0; m < RR_M; ++m) {
-    const float od = __shfl(d32, m);
-    const int orank = __shfl(rank, m);
-    const int oi = __shfl(my_i, m);
-    if (lane < RR_M && m != lane && sel >= 0 && oi != 0x7fffffff &&
-        fabsf(od - d32) <= RR_TOL * fmaxf(d32, 1e-30f) && (rank <= k || orank <= k))
-      flag = true;
+  __syncthreads();
+  // bitonic sort of (sd, si) by key
+  for (int s = 2; s <= P; s <<= 1)
+    for (int j = s >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < P; i += 64) {
+        const int p = i ^ j;
+        if (p > i) {
+          const bool up = (i & s) == 0;
+          const bool gt = key_less(sd[p], si[p], sd[i], si[i]);
+          if (gt == up) {
+            const float td = sd[i];
+            const int ti = si[i];
+            sd[i] = sd[p];
+            si[i] = si[p];
+            sd[p] = td;
+            si[p] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  // Near ties: runs of consecutive candidates each within tol (relative) of
+  // the previous one; a run that starts at or before rank k (it can move a
+  // candidate across the boundary or reorder the answer) is taken to FP64.
+  for (int i = lane; i < c; i += 64) {
+    sflag[i] = 0;
+    sx[i] = (double)sd[i];
   }
-  double d64 = (double)d32;
-  const uint64_t fl = __ballot(flag);
-  if (fl) {
-    for (int m = 0; m < RR_M; ++m) {
-      if (!((fl >> m) & 1ull)) continue;
-      const int r = __shfl(sel, m);
+  __syncthreads();
+  if (lane == 0) {
+    int nf = 0, start = 0;
+    for (int i = 1; i <= c; ++i) {
+      const bool tie = i < c && sd[i] - sd[i - 1] <= tol * sd[i];
+      if (!tie) {
+        if (i - start >= 2 && start <= k)
+          for (int m = start; m < i; ++m) {
+            sflag[m] = 1;
+            ++nf;
+          }
+        if (i > k) break;
+        start = i;
+      }
+    }
+    nflag_s = nf;
+  }
+  __syncthreads();
+  const int nflag = nflag_s;
+  if (nflag > 0) {
+    for (int m = 0; m < c; ++m) {
+      if (!sflag[m]) continue;  // uniform over the wave: sflag is in LDS
+      const float *rrow = Rf + (int64_t)si[m] * ldr;
       double s = 0.0;
       for (int j = lane; j < d; j += 64) {
-        const double t = (double)Qf[(int64_t)q * ldq + j] - (double)Rf[(int64_t)r * ldr + j];
+        const double t = (double)qrow[j] - (double)rrow[j];
         s += t * t;
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == m) d64 = s;
+      if (lane == 0) sx[m] = s;
     }
-    rank = 0;
-    for (int m = 0; m < RR_M; ++m) {
-      const double od = __shfl(d64, m);
-      const int oi = __shfl(my_i, m);
-      if (m != lane && key_less64(od, oi, d64, my_i)) ++rank;
+    __syncthreads();
+    // each flagged run sorted by (FP64 distance, index): runs are separated
+    // by relative gaps above tol, which FP32's error cannot cross
+    if (lane == 0) {
+      int m = 0;
+      while (m < c) {
+        if (!sflag[m]) {
+          ++m;
+          continue;
+        }
+        int e = m + 1;
+        while (e < c && sflag[e] && sd[e] - sd[e - 1] <= tol * sd[e]) ++e;
+        for (int x = m + 1; x < e; ++x) {  // insertion sort of [m, e)
+          const double xd = sx[x];
+          const float fd = sd[x];
+          const int xi = si[x];
+          int y = x - 1;
+          while (y >= m && key_less64(xd, xi, sx[y], si[y])) {
+            sx[y + 1] = sx[y];
+            sd[y + 1] = sd[y];
+            si[y + 1] = si[y];
+            --y;
+          }
+          sx[y + 1] = xd;
+          sd[y + 1] = fd;
+          si[y + 1] = xi;
+        }
+        m = e;
+      }
+      if (stats) {
+        atomicAdd(&stats[0], 1);
+        atomicAdd(&stats[1], nflag);
+      }
     }
-    if (stats && lane == 0) {
-      atomicAdd(&stats[0], 1);
-      atomicAdd(&stats[1], (int)__popcll(fl));
-    }
+    __syncthreads();
   }
-  if (lane < RR_M && sel >= 0 && rank < k) {
-    const float outd = flag ? (float)d64 : d32;
-    out_d[(int64_t)q * k + rank] = do_sqrt ? sqrtf(outd) : outd;
-    out_i[(int64_t)q * k + rank] = sel;
+  for (int j = lane; j < k && j < c; j += 64) {
+    const float outd = sflag[j] ? (float)sx[j] : sd[j];
+    out_d[(int64_t)q * k + j] = do_sqrt ? sqrtf(outd) : outd;
+    out_i[(int64_t)q * k + j] = si[j];
+  }
+  if (stats && lane == 0 && exact) atomicAdd(&stats[2], 1);
+  __syncthreads();  // the LDS arrays are reused by the next listed query
   }
 }
 
 int dpad_of(int d) { return ((d + 31) / 32) * 32; }
 
+struct Bf16Plan {
+  int dpad, ntiles, slices, slice, ck, nblk;
+  bool stream;
+  size_t bQb, bRb, bqn, brn, bmu, bpart, bmax, btmin, bthr, bcn, bex, bcd, bci, bovf, bbig,
+      bscr;
+  size_t bytes() const {
+    return bQb + bRb + bqn + brn + bmu + bpart + bmax + btmin + bthr + bcn + bex + bcd + bci +
+           bovf + bbig + bscr;
+  }
+};
+
+size_t a256(size_t b) { return (b + 255) / 256 * 256; }
+
+Bf16Plan bf16_plan(int Q, int R, int d, int k) {
+  Bf16Plan p;
+  p.dpad = dpad_of(d);
+  p.stream = p.dpad > 160;
+  p.ntiles = (R + CR - 1) / CR;
+  p.ck = k + CK_MARGIN;
+  p.nblk = (R + MEAN_ROWS - 1) / MEAN_ROWS;
+  const int step = p.stream ? SREF : CR;
+  const int steps = (R + step - 1) / step;
+  const int qblocks = (Q + CQ - 1) / CQ;
+  // reference slices: 8 (one per XCD), doubled until the grid has ~1024
+  // workgroups, never more than there are reference steps
+  int slices = 8;
+  while (slices < 64 && qblocks * slices < 1024) slices *= 2;
+  if (slices > steps) slices = steps;
+  p.slices = slices;
+  p.slice = ((R + slices - 1) / slices + step - 1) / step * step;
+  p.bQb = a256((size_t)Q * p.dpad * 4);
+  p.bRb = a256((size_t)R * p.dpad * 4);
+  p.bqn = a256((size_t)Q * 4);
+  p.brn = a256((size_t)R * 4);
+  p.bmu = a256((size_t)d * 4);
+  p.bpart = a256((size_t)p.nblk * d * 8);
+  p.bmax = 256;
+  p.btmin = a256((size_t)Q * p.ntiles * 4);
+  p.bthr = a256((size_t)Q * 4);
+  p.bcn = a256((size_t)Q * 4);
+  p.bex = a256((size_t)Q * 4);
+  p.bcd = a256((size_t)Q * CAND_CAP * 4);
+  p.bci = a256((size_t)Q * CAND_CAP * 4);
+  p.bovf = a256((size_t)Q * 4 + 4);
+  p.bbig = a256((size_t)Q * 4 + 4);
+  p.bscr = a256((size_t)FB_GRID * R * 4);
+  return p;
+}
+
 }  // namespace
 
-hipError_t launch_knn_select(const float *dist, int Q, int R, int k, float *out_d, int *out_i,
-                             int do_sqrt, hipStream_t s) {
-  const dim3 grid((Q + SEL_Q - 1) / SEL_Q);
-  if (k <= 16)
-    knn_select_kernel<16><<<grid, SEL_Q * sel_parts<16>(), 0, s>>>(dist, Q, R, k, out_d, out_i,
-                                                                   do_sqrt);
-  else
-    knn_select_kernel<SEL_KMAX><<<grid, SEL_Q * sel_parts<SEL_KMAX>(), 0, s>>>(
-        dist, Q, R, k, out_d, out_i, do_sqrt);
-  return hipGetLastError();
-}
+int knn_bf16_kmax() { return KMAX; }
 
-size_t knn_bf16_workspace(int Q, int R, int d, int *slices_out) {
-  const int dpad = dpad_of(d);
-  // 8 reference slices, one per XCD (see knn_pass_kernel), fewer when the
-  // references are fewer than 8 tiles.
-  int slices = 8;
-  const int max_slices = (R + CR - 1) / CR;
-  if (slices > max_slices) slices = max_slices < 1 ? 1 : max_slices;
-  if (slices_out) *slices_out = slices;
-  const size_t ntiles = ((size_t)R + CR - 1) / CR;
-  auto a256 = [](size_t b) { return (b + 255) / 256 * 256; };
-  size_t bytes = 0;
-  bytes += a256((size_t)Q * dpad * 4);  // hi | lo BF16
-  bytes += a256((size_t)R * dpad * 4);
-  bytes += a256((size_t)Q * 4);          // query norms
-  bytes += a256((size_t)R * 4);          // reference norms
-  bytes += a256((size_t)Q * ntiles * 4); // tile minima
-  bytes += a256((size_t)Q * 4);          // tau
-  bytes += a256((size_t)Q * 4);          // candidate counts
-  bytes += 2 * a256((size_t)Q * CAND_CAP * 4);
-  return bytes;
-}
+size_t knn_bf16_workspace(int Q, int R, int d) { return bf16_plan(Q, R, d, 1).bytes(); }
 
 hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, int64_t ldr, int Q,
                            int R, int d, int k, float *out_d, int *out_i, int do_sqrt,
                            void *workspace, int *stats, hipStream_t s) {
-  int slices = 1;
-  knn_bf16_workspace(Q, R, d, &slices);
-  const int dpad = dpad_of(d);
-  const int ntiles = (R + CR - 1) / CR;
+  const Bf16Plan p = bf16_plan(Q, R, d, k);
   char *w = static_cast<char *>(workspace);
   auto take = [&](size_t bytes) {
-    char *p = w;
-    w += (bytes + 255) / 256 * 256;
-    return p;
+    char *ptr = w;
+    w += bytes;
+    return ptr;
   };
-  __bf16 *Qb = reinterpret_cast<__bf16 *>(take((size_t)Q * dpad * 4));
-  __bf16 *Rb = reinterpret_cast<__bf16 *>(take((size_t)R * dpad * 4));
-  float *qn = reinterpret_cast<float *>(take((size_t)Q * 4));
-  float *rn = reinterpret_cast<float *>(take((size_t)R * 4));
-  float *tmin = reinterpret_cast<float *>(take((size_t)Q * ntiles * 4));
-  float *tau = reinterpret_cast<float *>(take((size_t)Q * 4));
-  int *cn = reinterpret_cast<int *>(take((size_t)Q * 4));
-  float *cd = reinterpret_cast<float *>(take((size_t)Q * CAND_CAP * 4));
-  int *ci = reinterpret_cast<int *>(take((size_t)Q * CAND_CAP * 4));
+  __bf16 *Qb = reinterpret_cast<__bf16 *>(take(p.bQb));
+  __bf16 *Rb = reinterpret_cast<__bf16 *>(take(p.bRb));
+  float *qn = reinterpret_cast<float *>(take(p.bqn));
+  float *rn = reinterpret_cast<float *>(take(p.brn));
+  float *mu = reinterpret_cast<float *>(take(p.bmu));
+  double *part = reinterpret_cast<double *>(take(p.bpart));
+  unsigned *rnmax = reinterpret_cast<unsigned *>(take(p.bmax));
+  float *tmin = reinterpret_cast<float *>(take(p.btmin));
+  float *thr = reinterpret_cast<float *>(take(p.bthr));
+  int *cn = reinterpret_cast<int *>(take(p.bcn));
+  int *cex = reinterpret_cast<int *>(take(p.bex));
+  float *cd = reinterpret_cast<float *>(take(p.bcd));
+  int *ci = reinterpret_cast<int *>(take(p.bci));
+  int *ovf = reinterpret_cast<int *>(take(p.bovf));
+  int *big = reinterpret_cast<int *>(take(p.bbig));
+  float *scr = reinterpret_cast<float *>(take(p.bscr));
 
-  prep_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, Q, d, dpad, Qb, qn);
-  prep_kernel<<<(R + 3) / 4, 256, 0, s>>>(ref, ldr, R, d, dpad, Rb, rn);
-  int slice = (R + slices - 1) / slices;
-  slice = (slice + CR - 1) / CR * CR;
-  const dim3 grid(((Q + CQ - 1) / CQ) * slices);
+  mean_partial_kernel<<<p.nblk, 256, 0, s>>>(ref, ldr, R, d, part);
+  mean_final_kernel<<<(d + 255) / 256, 256, 0, s>>>(part, p.nblk, d, R, mu, rnmax);
+  prep_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, Q, d, p.dpad, mu, Qb, qn, nullptr);
+  prep_kernel<<<(R + 3) / 4, 256, 0, s>>>(ref, ldr, R, d, p.dpad, mu, Rb, rn, rnmax);
+  const PassArgs a{qn, rn, thr, tmin, p.ntiles, cd, ci, cn, ovf, ovf + 1};
+  const dim3 grid(((Q + CQ - 1) / CQ) * p.slices);
+  // E_q = e_coef (||q - mu||^2 + max ||r - mu||^2): see the bound at the top
+  const float e_coef = 1.25f * (3.1f * ldexpf(1.0f, -16) + (float)(3 * p.dpad + d / 64 + 8) *
+                                                             ldexpf(1.0f, -24));
   for (int pass = 1; pass <= 2; ++pass) {
+    if (p.stream) {
+      if (pass == 1)
+        knn_pass_stream_kernel<1><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.dpad, p.slices, p.slice, a);
+      else
+        knn_pass_stream_kernel<2><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.dpad, p.slices, p.slice, a);
+    } else {
 #define SIMD2_KNN_PASS(D)                                                                      \
   case D:                                                                                      \
     if (pass == 1)                                                                             \
-      knn_pass_kernel<D, 1><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slices, slice, tmin,  \
-                                                 ntiles, tau, cd, ci, cn);                     \
+      knn_pass_kernel<D, 1><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.slices, p.slice, a);         \
     else                                                                                       \
-      knn_pass_kernel<D, 2><<<grid, 256, 0, s>>>(Qb, Rb, qn, rn, Q, R, slices, slice, tmin,  \
-                                                 ntiles, tau, cd, ci, cn);                     \
+      knn_pass_kernel<D, 2><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.slices, p.slice, a);         \
     break;
-    switch (dpad) {
-      SIMD2_KNN_PASS(32)
-      SIMD2_KNN_PASS(64)
-      SIMD2_KNN_PASS(96)
-      SIMD2_KNN_PASS(128)
-      SIMD2_KNN_PASS(160)
-    default: return hipErrorInvalidValue;
-    }
+      switch (p.dpad) {
+        SIMD2_KNN_PASS(32)
+        SIMD2_KNN_PASS(64)
+        SIMD2_KNN_PASS(96)
+        SIMD2_KNN_PASS(128)
+        SIMD2_KNN_PASS(160)
+      default: return hipErrorInvalidValue;
+      }
 #undef SIMD2_KNN_PASS
-    if (pass == 1) knn_tau_kernel<<<(Q + 3) / 4, 256, 0, s>>>(tmin, ntiles, Q, tau, cn);
+    }
+    if (pass == 1)
+      knn_tau_kernel<<<(Q + 3) / 4, 256, 0, s>>>(tmin, p.ntiles, Q, p.ck, e_coef, qn, rnmax, thr,
+                                                 cn, cex, ovf, big);
   }
-  rerank_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, ref, ldr, Q, d, k, cd, ci, cn, CAND_CAP,
-                                           out_d, out_i, do_sqrt, stats);
+  knn_overflow_kernel<<<FB_GRID, 256, 0, s>>>(query, ldq, ref, ldr, R, d, ovf, ovf + 1, scr, cd,
+                                              ci, cn, cex);
+  // relative gap below which FP32 may order two distances wrongly: the
+  // canonical sum's error is at most about (d / 8 + 3) units of 2^-24 each side
+  const float tol = fmaxf(1e-5f, (float)(d / 4 + 8) * ldexpf(1.0f, -24));
+  rerank_small_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, ref, ldr, Q, d, k, tol, cd, ci, cn,
+                                                   cex, out_d, out_i, do_sqrt, stats, big, big + 1);
+  rerank_big_kernel<<<BIG_GRID_RR, 64, 0, s>>>(query, ldq, ref, ldr, d, k, tol, cd, ci, cn, cex,
+                                               out_d, out_i, do_sqrt, stats, big, big + 1);
   return hipGetLastError();
 }
 

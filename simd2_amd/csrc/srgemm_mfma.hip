@@ -9,6 +9,11 @@
 // lane l supplies A[l%16][l/16] and B[l/16][l%16]; accumulator element v of
 // lane l is D[4*(l/16)+v][l%16].  LDS rows are padded by 16 floats so the
 // four k-rows a wave reads at once land in distinct banks.
+//
+// Every shape runs here: VEC loads operands as float4 (leading dimensions
+// and bases 16-byte aligned, K and N multiples of 4), otherwise one float at
+// a time; rows, columns and k past the matrix are read as zeros and never
+// stored (a zero contributes an exact +0 to every sum it enters).
 #include "internal.hpp"
 
 namespace simd2 {
@@ -19,6 +24,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MB = 128, NB = 128, KB = 16, LDP = 16;
 
+template <bool VEC>
 __global__ __launch_bounds__(256) void plusmul_mfma_kernel(
     int R, int Cn, int K, const float *__restrict__ X, int64_t ldx,
     const float *__restrict__ Y, int64_t ldy, float *O, int64_t ldo,
@@ -38,14 +44,32 @@ __global__ __launch_bounds__(256) void plusmul_mfma_kernel(
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  // Slab loads: X 128 rows x 16 k (2 float4 per thread), Y 16 x 128 (2 each).
+  // Slab loads: X 128 rows x 16 k (2 x 4 floats per thread), Y 16 x 128 (2 x 4).
   const int xr = tid / 4, xk = (tid % 4) * 4;
   const int yk = tid / 32, yc = (tid % 32) * 4;
-  const float *xp0 = X + (int64_t)(row0 + xr) * ldx + xk;
-  const float *xp1 = xp0 + 64 * ldx;
-  const float *yp0 = Y + (int64_t)yk * ldy + col0 + yc;
-  const float *yp1 = yp0 + 8 * ldy;
   float4 x0, x1, y0, y1;
+  // 4 consecutive floats of row r of a matrix from column c; zeros past (rows, cols)
+  auto ld4 = [&](const float *M, int64_t ld, int r, int c, int rows, int cols) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (r < rows) {
+      const float *p = M + (int64_t)r * ld + c;
+      if (VEC) {
+        if (c < cols) v = *reinterpret_cast<const float4 *>(p);
+      } else {
+        if (c < cols) v.x = p[0];
+        if (c + 1 < cols) v.y = p[1];
+        if (c + 2 < cols) v.z = p[2];
+        if (c + 3 < cols) v.w = p[3];
+      }
+    }
+    return v;
+  };
+  auto fetch = [&](int k0) {
+    x0 = ld4(X, ldx, row0 + xr, k0 + xk, R, K);
+    x1 = ld4(X, ldx, row0 + xr + 64, k0 + xk, R, K);
+    y0 = ld4(Y, ldy, k0 + yk, col0 + yc, K, Cn);
+    y1 = ld4(Y, ldy, k0 + yk + 8, col0 + yc, K, Cn);
+  };
   auto stage = [&](int b) {
     Xs[b][xk + 0][xr] = x0.x; Xs[b][xk + 1][xr] = x0.y;
     Xs[b][xk + 2][xr] = x0.z; Xs[b][xk + 3][xr] = x0.w;
@@ -54,12 +78,9 @@ __global__ __launch_bounds__(256) void plusmul_mfma_kernel(
     *reinterpret_cast<float4 *>(&Ys[b][yk][yc]) = y0;
     *reinterpret_cast<float4 *>(&Ys[b][yk + 8][yc]) = y1;
   };
-  const int nkt = K / KB;
+  const int nkt = (K + KB - 1) / KB;
   if (nkt > 0) {
-    x0 = *reinterpret_cast<const float4 *>(xp0);
-    x1 = *reinterpret_cast<const float4 *>(xp1);
-    y0 = *reinterpret_cast<const float4 *>(yp0);
-    y1 = *reinterpret_cast<const float4 *>(yp1);
+    fetch(0);
     stage(0);
   }
   __syncthreads();
@@ -67,13 +88,7 @@ __global__ __launch_bounds__(256) void plusmul_mfma_kernel(
   for (int kt = 0; kt < nkt; ++kt) {
     const int buf = kt & 1;
     const bool more = kt + 1 < nkt;
-    if (more) {
-      xp0 += KB; xp1 += KB; yp0 += KB * ldy; yp1 += KB * ldy;
-      x0 = *reinterpret_cast<const float4 *>(xp0);
-      x1 = *reinterpret_cast<const float4 *>(xp1);
-      y0 = *reinterpret_cast<const float4 *>(yp0);
-      y1 = *reinterpret_cast<const float4 *>(yp1);
-    }
+    if (more) fetch((kt + 1) * KB);
 #pragma unroll
     for (int ks = 0; ks < KB; ks += 4) {
       float a[4], b[4];
@@ -99,6 +114,7 @@ __global__ __launch_bounds__(256) void plusmul_mfma_kernel(
       for (int v = 0; v < 4; ++v) {
         const int r = row0 + wr * 64 + i * 16 + 4 * lk + v;
         const int c = col0 + wc * 64 + j * 16 + lr;
+        if (r >= R || c >= Cn) continue;
         float o = acc[i][j][v];
         if (C) o = o + C[(int64_t)r * ldc + c];
         O[(int64_t)r * ldo + c] = o;
@@ -112,17 +128,16 @@ hipError_t launch_plusmul_mfma(int R, int Cn, int K, const float *X,
                                float *O, int64_t ldo, const float *C,
                                int64_t ldc, hipStream_t s) {
   auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-  const bool ok = R % MB == 0 && Cn % NB == 0 && K % KB == 0 && ldx % 4 == 0 &&
-                  ldy % 4 == 0 && aligned(X) && aligned(Y);
-  if (!ok) {  // shapes off the 128x128x16 grid use the VALU kernel
-    Epilogue e;
-    e.C = C;
-    e.ldc = ldc;
-    return launch_valu_f32(1, R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
-  }
-  const int tiles_c = Cn / NB;
-  plusmul_mfma_kernel<<<dim3((R / MB) * tiles_c), dim3(256), 0, s>>>(
-      R, Cn, K, X, ldx, Y, ldy, O, ldo, C, ldc, tiles_c);
+  const bool vec = K % 4 == 0 && Cn % 4 == 0 && ldx % 4 == 0 && ldy % 4 == 0 && aligned(X) &&
+                   aligned(Y);
+  const int tiles_c = (Cn + NB - 1) / NB;
+  const dim3 grid(((R + MB - 1) / MB) * tiles_c);
+  if (vec)
+    plusmul_mfma_kernel<true><<<grid, dim3(256), 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo, C, ldc,
+                                                         tiles_c);
+  else
+    plusmul_mfma_kernel<false><<<grid, dim3(256), 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo, C, ldc,
+                                                          tiles_c);
   return hipGetLastError();
 }
 

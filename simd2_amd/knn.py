@@ -17,7 +17,9 @@ def _bind():
     if getattr(L, "_knn_bound", False):
         return L
     i, p = ctypes.c_int, ctypes.c_void_p
-    L.simd2_knn_select.argtypes = [p, i, i, i, p, p, i, p]
+    L.simd2_knn_select_workspace_size.argtypes = [i, i, i]
+    L.simd2_knn_select_workspace_size.restype = ctypes.c_size_t
+    L.simd2_knn_select.argtypes = [p, i, i, i, p, p, i, p, ctypes.c_size_t, p]
     L.simd2_knn_select.restype = i
     L.simd2_knn_bf16_workspace_size.argtypes = [i, i, i]
     L.simd2_knn_bf16_workspace_size.restype = ctypes.c_size_t
@@ -31,17 +33,30 @@ def _bind():
     return L
 
 
-def knn_select(dist_rq, k, sqrt=True, stream=None):
+def select_workspace_bytes(Q, R, k):
+    return int(_bind().simd2_knn_select_workspace_size(Q, R, k))
+
+
+def knn_select(dist_rq, k, sqrt=True, stream=None, workspace=None):
     """Top-k per query of a distance matrix in the reference layout
     (dist_rq: R x Q device tensor, row r = distances of all queries to
-    reference r).  Returns (dist k x Q, index k x Q)."""
+    reference r), any 1 <= k <= R.  ``workspace`` (uint8 device tensor) is
+    allocated when not given.  Returns (dist k x Q, index k x Q)."""
     torch = _torch()
     R, Q = dist_rq.shape
     out_d = torch.empty((k, Q), dtype=torch.float32, device=dist_rq.device)
     out_i = torch.empty((k, Q), dtype=torch.int32, device=dist_rq.device)
+    need = select_workspace_bytes(Q, R, k)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=dist_rq.device)
     st = _bind().simd2_knn_select(_ptr(dist_rq), Q, R, k, _ptr(out_d), _ptr(out_i),
-                                  SQRT if sqrt else 0, _stream(stream))
+                                  SQRT if sqrt else 0, _ptr(workspace), workspace.numel(),
+                                  _stream(stream))
     _check(st, "simd2_knn_select")
+    if stream is not None:
+        s = stream if isinstance(stream, torch.cuda.Stream) else torch.cuda.ExternalStream(stream)
+        for t in (workspace, out_d, out_i):
+            t.record_stream(s)
     return out_d, out_i
 
 
@@ -69,7 +84,7 @@ class KnnBF16:
         self.Q, self.R, self.dim = Q, R, dim
         nbytes = _bind().simd2_knn_bf16_workspace_size(Q, R, dim)
         self.ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
-        self.stats = torch.zeros(2, dtype=torch.int32, device=device)
+        self.stats = torch.zeros(3, dtype=torch.int32, device=device)
 
     def __call__(self, query, ref, k, sqrt=False, out=None, stream=None):
         torch = _torch()
@@ -88,8 +103,10 @@ class KnnBF16:
 
 
 def knn_bf16(query, ref, k, sqrt=False, stream=None):
-    """One-shot precision-form kNN.  Returns (dist Q x k, index Q x k, stats)
-    where stats = [queries that used FP64, candidates recomputed in FP64]."""
+    """One-shot precision-form kNN (any dim, 1 <= k <= 256).  Returns
+    (dist Q x k, index Q x k, stats) where stats = [queries that used FP64,
+    candidates recomputed in FP64, queries whose candidates overflowed and
+    were found by the exact scan of every reference]."""
     torch = _torch()
     runner = KnnBF16(query.shape[0], ref.shape[0], query.shape[1], device=query.device)
     d, i = runner(query, ref, k, sqrt=sqrt, stream=stream)

@@ -8,7 +8,10 @@ this library's two converge-loop structures, run through the app driver
 loop (squaring, separate change test, a flag read per squaring) and the loop
 with the change test fused into the squaring.  Both must give the same
 iteration count and the same checksum (utils/check_sum.h:3-18), which the
-table shows.  Inputs are synthetic, in each app's input format:
+table shows.  With --reference-inputs the rows are the reference table's own (its input
+files that exist in its tree, copied under tests/golden/graphs/reference/;
+GTC and PLD from the drivers' generators with the table's arguments).
+Otherwise inputs are synthetic, in each app's input format:
 
   APSP / MCP / MinRP / MaxRP  directed Erdos-Renyi edge lists on stdin
                               (apps/apsp/cuASR/apsp_cuASR.cu:78-91), weights
@@ -105,6 +108,99 @@ def run_app(app, v, p, loop, reps, seed=1, binary=APP_BIN):
     return parse(res.stdout)
 
 
+REFERENCE_DATA = os.path.join(os.path.dirname(HERE), "tests", "golden", "graphs", "reference")
+# The rows of the reference's table (scripts/run_apps.py:100-361) whose input
+# files exist in its tree: (app, table name, file); GTC and PLD use the
+# drivers' generators with the reference's arguments.
+REFERENCE_ROWS = [("aplp", "APLP-Small", "DAG_rmat_4096.txt"),
+                  ("aplp", "APLP-Medium", "DAG_rmat_8192.txt"),
+                  ("mst", "MST-Small", "mst_rmat_1024.txt"),
+                  ("minrp", "MINRP-Small", "minrp_rmat_4096.txt"),
+                  ("mcp", "MCP-1024", "parsed_rmat_1024.txt"),
+                  ("mcp", "MCP-2048", "parsed_rmat_2048.txt"),
+                  ("apsp", "APSP-1024", "parsed_rmat_1024.txt"),
+                  ("apsp", "APSP-2048", "parsed_rmat_2048.txt")]
+REFERENCE_GTC = [("GTC-SMALL", 1024, 0.0001), ("GTC-Medium", 4096, 0.0001),
+                 ("GTC-Large", 8192, 0.0001)]
+REFERENCE_PLD = [("KNN-SMALL", 4096, 10), ("KNN-Medium", 8192, 10)]
+
+
+def run_app_file(app, path, loop, reps, binary=APP_BIN):
+    """(mean ms per solve, iterations, checksum) of simd2_app on a reference
+    input file (gzip-compressed edge list or PBBS file)."""
+    import gzip
+    with gzip.open(path, "rt") as f:
+        data = f.read()
+    cmd = [binary, app, "-f"]
+    stdin, tmp = None, None
+    if app == "mst":
+        fd, tmp = tempfile.mkstemp(suffix=".pbbs")
+        with os.fdopen(fd, "w") as f:
+            f.write(data)
+        cmd.append(tmp)
+    else:
+        stdin = data
+    cmd += ["--reps", str(reps), "--loop", loop]
+    try:
+        res = subprocess.run(cmd, input=stdin, capture_output=True, text=True, check=True)
+    finally:
+        if tmp:
+            os.unlink(tmp)
+    return parse(res.stdout)
+
+
+def run_pld_uniform(v, k, reps=3):
+    """The reference's PLD row: knn_mxu with rn = qn = d = v and k, on
+    coordinates uniform in [0, 10) as its test generates them
+    (apps/pld/cuASR/test.cpp); returns the distance kernel's ms inside
+    knn_mxu (what the reference's kernel_time measures) and the whole call's."""
+    import time
+    from .knn import knn_mxu
+    rng = np.random.default_rng(v)
+    ref = (10.0 * rng.random((v, v))).astype(np.float32)
+    qry = (10.0 * rng.random((v, v))).astype(np.float32)
+    knn_mxu(ref[:256], qry[:, :256], k)  # warm-up
+    kern, wall = 0.0, 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, _, t = knn_mxu(ref, qry, k)
+        wall += time.perf_counter() - t0
+        kern += t
+    return kern / reps * 1e3, wall / reps * 1e3
+
+
+def reference_table(reps, data=REFERENCE_DATA, pld=True):
+    """The reference table's rows on the reference's own inputs."""
+    rows = []
+    print("Application Datasize reference-loop(ms) fused-loop(ms) iterations same-result",
+          flush=True)
+    for app, name, fname in REFERENCE_ROWS:
+        path = os.path.join(data, fname + ".gz")
+        r_ms, r_it, r_cs = run_app_file(app, path, "reference", reps)
+        f_ms, f_it, f_cs = run_app_file(app, path, "fused", reps)
+        same = r_it == f_it and r_cs == f_cs
+        rows.append({"row": name, "app": NAMES[app], "input": fname,
+                     "reference_loop_ms": round(r_ms, 3), "fused_loop_ms": round(f_ms, 3),
+                     "iterations": r_it, "checksum": r_cs, "same_result": same})
+        print(f"{name}: {fname} {r_ms:.3f} {f_ms:.3f} {r_it} {same}", flush=True)
+    for name, v, p in REFERENCE_GTC:
+        r_ms, r_it, r_cs = run_app("gtc", v, p, "reference", reps)
+        f_ms, f_it, f_cs = run_app("gtc", v, p, "fused", reps)
+        same = r_it == f_it and r_cs == f_cs
+        rows.append({"row": name, "app": "GTC", "input": f"rgg_1d_directed({v}, {p}), seed 7",
+                     "reference_loop_ms": round(r_ms, 3), "fused_loop_ms": round(f_ms, 3),
+                     "iterations": r_it, "checksum": r_cs, "same_result": same})
+        print(f"{name}: {v} {r_ms:.3f} {f_ms:.3f} {r_it} {same}", flush=True)
+    if pld:
+        for name, v, k in REFERENCE_PLD:
+            kern, wall = run_pld_uniform(v, k, reps)
+            rows.append({"row": name, "app": "PLD", "input": f"rn = qn = d = {v}, k = {k}",
+                         "knn_mxu_distance_kernel_ms": round(kern, 3),
+                         "knn_mxu_call_ms": round(wall, 3)})
+            print(f"{name}: {v}^3 k={k} distances {kern:.3f} whole call {wall:.3f}", flush=True)
+    return rows
+
+
 def run_pld(v, reps=3):
     """knn_mxu and the precision form for v queries x v references: the
     distance kernel's ms inside knn_mxu (what the reference's kernel_time
@@ -144,9 +240,17 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--apps", nargs="+", default=APPS + ["pld"])
     ap.add_argument("--json", default=None, help="also write the rows here")
+    ap.add_argument("--reference-inputs", action="store_true",
+                    help="the reference table's rows on its own input files instead")
     args = ap.parse_args(argv)
     if not os.path.exists(APP_BIN):
         sys.exit(f"{APP_BIN} is missing: build it with make -C simd2_amd/csrc")
+    if args.reference_inputs:
+        rows = reference_table(args.reps)
+        if args.json:
+            with open(args.json, "w") as f:
+                json.dump(rows, f, indent=1)
+        return rows
     rows = []
     print("Application Datasize reference-loop(ms) fused-loop(ms) iterations same-result",
           flush=True)

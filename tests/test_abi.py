@@ -150,18 +150,33 @@ def test_converge_argument_errors(lib):
 def test_knn_argument_errors(lib):
     import simd2_amd.knn as K
     L = K._bind()
-    assert L.simd2_knn_select(P1, 100, 50, 0, P2, P3, 0, None) == simd2_amd.ERROR_INVALID_PROBLEM
-    assert L.simd2_knn_select(P1, 100, 50, 33, P2, P3, 0, None) == simd2_amd.ERROR_INVALID_PROBLEM
-    assert L.simd2_knn_select(P1, 100, 10, 16, P2, P3, 0, None) == simd2_amd.ERROR_INVALID_PROBLEM
-    assert L.simd2_knn_select(None, 100, 50, 16, P2, P3, 0, None) == simd2_amd.ERROR_NULL_POINTER
+    sel = L.simd2_knn_select
+    need = L.simd2_knn_select_workspace_size(100, 50, 33)
+    assert need > 0
+    assert sel(P1, 100, 50, 0, P2, P3, 0, P4, need, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert sel(P1, 100, 50, 51, P2, P3, 0, P4, need, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert sel(P1, 100, 10, 16, P2, P3, 0, P4, need, None) == simd2_amd.ERROR_INVALID_PROBLEM
+    assert sel(None, 100, 50, 16, P2, P3, 0, P4, need, None) == simd2_amd.ERROR_NULL_POINTER
+    # any k <= ref_nb is accepted; the workspace is checked before any launch
+    assert sel(P1, 100, 50, 33, P2, P3, 0, None, 0, None) == simd2_amd.ERROR_WORKSPACE
+    assert sel(P1, 100, 50, 50, P2, P3, 0, P4,
+               L.simd2_knn_select_workspace_size(100, 50, 50) - 1, None) == simd2_amd.ERROR_WORKSPACE
+    # queries are batched: the workspace stays bounded at the benchmark's size and for huge k
+    assert L.simd2_knn_select_workspace_size(16384, 16384, 16) < (300 << 20)
+    assert L.simd2_knn_select_workspace_size(16384, 16384, 16384) < (300 << 20)
+    assert L.simd2_knn_select_workspace_size(10, 20, 21) == 0
     ws = L.simd2_knn_bf16_workspace_size(1000, 2000, 128)
     assert ws > 1000 * 128 * 2 + 2000 * 128 * 2
     args = (P1, 128, P2, 128, 1000, 2000, 128, 16, P3, P4, 0)
     assert L.simd2_knn_bf16(*args, P4 + (1 << 20), ws - 1, None, None) == simd2_amd.ERROR_WORKSPACE
-    assert L.simd2_knn_bf16(P1, 128, P2, 128, 1000, 2000, 200, 16, P3, P4, 0, P4, ws, None,
-                            None) == simd2_amd.ERROR_INVALID_PROBLEM   # dim > 160
-    assert L.simd2_knn_bf16(P1, 128, P2, 128, 1000, 2000, 128, 32, P3, P4, 0, P4, ws, None,
-                            None) == simd2_amd.ERROR_INVALID_PROBLEM   # k > 31
+    assert L.simd2_knn_bf16(P1, 128, P2, 128, 1000, 2000, 128, 257, P3, P4, 0, P4, ws, None,
+                            None) == simd2_amd.ERROR_INVALID_PROBLEM   # k > 256
+    assert L.simd2_knn_bf16(P1, 128, P2, 128, 1000, 20, 128, 21, P3, P4, 0, P4, ws, None,
+                            None) == simd2_amd.ERROR_INVALID_PROBLEM   # k > ref_nb
+    assert L.simd2_knn_bf16(P1, 100, P2, 128, 1000, 2000, 128, 16, P3, P4, 0, P4, ws, None,
+                            None) == simd2_amd.ERROR_INVALID_PROBLEM   # ldq < dim
+    # any dim: the workspace grows with it
+    assert L.simd2_knn_bf16_workspace_size(1000, 2000, 4096) > 3 * ws
 
 
 def test_batch_argument_errors(lib):

@@ -1,10 +1,9 @@
 """GPU parity of the semiring GEMM entry points against the oracle.
 
-Bit-exact for the six min/max semirings and or_and (and for plus_mul /
-minus_square where they take the sequential VALU path on shapes off the
-128x128x16 grid, and for minus_square on integer-valued data); plus_mul on
-MFMA and the fused minus_square on fractional data within 1e-5 relative of
-FP64 (tolerance stated per test)."""
+Bit-exact for the six min/max semirings and or_and (and for minus_square on
+integer-valued data); plus_mul (on the matrix cores for every shape) and
+the fused minus_square on fractional data within 1e-5 relative of FP64
+(tolerance stated per test)."""
 import sys
 
 import numpy as np
@@ -44,6 +43,20 @@ def fp64_ref(op, A, B, C, epi):
     return R + C.astype(np.float64) if epi else R
 
 
+def dot_err(D, A, B, C, epi):
+    """max |D - FP64 product| / (|A| |B| [+ |C|]): the error of a float sum
+    relative to the sum of its terms' magnitudes (order-independent scale)."""
+    if D.size == 0:
+        return 0.0
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    ref = A64 @ B64
+    scale = np.abs(A64) @ np.abs(B64)
+    if epi:
+        ref = ref + C.astype(np.float64)
+        scale = scale + np.abs(C.astype(np.float64))
+    return float(np.max(np.abs(D.astype(np.float64) - ref) / np.maximum(scale, 1e-30)))
+
+
 def test_golden_cases_all_ops(gpu, golden_srgemm):
     """The reference test's inputs and the reference host_srgemm's outputs."""
     g = golden_srgemm
@@ -56,7 +69,10 @@ def test_golden_cases_all_ops(gpu, golden_srgemm):
             if op == "minus_square":
                 # fused (a-b)^2 + acc: one rounding fewer than the oracle
                 np.testing.assert_allclose(D, want, rtol=1e-5, atol=1e-6)
-            else:  # includes plus_mul: these shapes take the sequential VALU path
+            elif op == "plus_mul":
+                # MFMA accumulation order: within 1e-5 of the sum of |terms|
+                assert dot_err(D, A, B, C, epi) <= 1e-5, (tag, op)
+            else:
                 assert np.array_equal(bits(D), bits(want)), (tag, op)
         Ai, Bi, Ci = ((x.astype(np.int32) % 2).astype(np.int32) for x in (A, B, C))
         say(f"golden {tag} or_and")
@@ -94,7 +110,7 @@ def test_tile_grid_exact_ops(gpu, oracle_mod, M, N, K):
 
 @pytest.mark.parametrize("M,N,K", [(256, 256, 256), (512, 384, 1024), (200, 300, 100)])
 def test_plus_mul_and_minus_square_tolerance(gpu, M, N, K):
-    """plus_mul (MFMA on the grid, VALU off it) and minus_square (fused FMA)
+    """plus_mul (MFMA) and minus_square (fused FMA)
     within 1e-5 relative of FP64; inputs U{1..100} without holes."""
     for op in ("plus_mul", "minus_square"):
         for epi in (False, True):
@@ -135,6 +151,36 @@ def test_plus_mul_mfma_fractional(gpu):
     D = run(gpu, "plus_mul", A, B, C, True).astype(np.float64)
     ref = A.astype(np.float64) @ B.astype(np.float64) + C
     assert np.max(np.abs(D - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("M,N,K,pad", [(1000, 777, 300, 0), (130, 70, 65, 3), (1, 5, 3, 1),
+                                       (257, 129, 1, 0), (64, 300, 17, 2), (128, 128, 16, 1)])
+def test_plus_mul_mfma_every_shape(gpu, M, N, K, pad):
+    """plus_mul runs on the matrix cores for every shape: off the 128x128x16
+    grid, K not a multiple of 4, and leading dimensions padded by `pad`
+    floats (unaligned rows take the element-wise loads)."""
+    import simd2_amd
+    torch = gpu
+    rng = np.random.default_rng(M + N + K + pad)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    C = rng.standard_normal((M, N)).astype(np.float32)
+    Ap = np.zeros((M, K + pad), np.float32)
+    Bp = np.zeros((K, N + pad), np.float32)
+    Cp = np.zeros((M, N + pad), np.float32)
+    Ap[:, :K], Bp[:, :N], Cp[:, :N] = A, B, C
+    tA, tB, tC = (torch.from_numpy(x).cuda() for x in (Ap, Bp, Cp))
+    tD = torch.full((M, N + pad), 7.0, device="cuda")
+    for epi in (False, True):
+        say(f"plus_mul mfma {M}x{N}x{K} pad={pad} epi={epi}")
+        # row-major A @ B as the column-major call (N, M, K, B, ldb, A, lda, C, ldc, D)
+        st = simd2_amd.cuasr_srsgemm("plus_mul", N, M, K, tB, N + pad, tA, K + pad, tC, N + pad,
+                                     tD, epi)
+        assert st == 0
+        torch.cuda.synchronize()
+        D = tD.cpu().numpy()
+        assert dot_err(D[:, :N], A, B, C, epi) <= 1e-5
+        assert np.all(D[:, N:] == 7.0)  # padding untouched
 
 
 def test_large_rows_sampled(gpu, oracle_mod):
