@@ -181,6 +181,23 @@ int simd2_minus_square_exact(int M, int N, int K, const float *A, int lda, const
                              int do_epilogue, void *workspace, size_t workspace_bytes,
                              hipStream_t stream);
 
+/* simd2_minus_square_exact without a workspace argument and without any
+ * allocation (no counterpart in the reference): for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The workspace is a scratch array
+ * the library holds on each device it runs on (SIMD2_MSQ_NOALLOC_MAX_BYTES,
+ * kept until the process ends); a problem whose
+ * simd2_minus_square_workspace(M, N, K) exceeds it is refused with
+ * SIMD2_ERROR_WORKSPACE (4096 x 4096 x 4096 needs about 32 MiB and fits).
+ * Results equal simd2_minus_square_exact's bit for bit.  Every call on one
+ * device shares that device's scratch: calls that can overlap on one device
+ * (different streams, different host threads) race and must be ordered by
+ * the caller. */
+#define SIMD2_MSQ_NOALLOC_MAX_BYTES ((size_t)40 << 20)
+int simd2_minus_square_exact_noalloc(int M, int N, int K, const float *A, int lda,
+                                     const float *B, int ldb, const float *C, int ldc, float *D,
+                                     int ldd, int do_epilogue, hipStream_t stream);
+
 /* 64x64 tiles of the last simd2_minus_square_exact call on this workspace
  * and problem that ran on the int8 matrix cores and on the direct kernel
  * (for tests and reports).  Synchronises `stream`. */

@@ -67,6 +67,7 @@ EXPORTS = list(DROPIN.values()) + [
     "tensor_srgemm", "simd2_tensor_srgemm", "f2h_device", "h2f_device",
     "simd2_srgemm_ex", "simd2_tile_config_count", "simd2_pick_tile_config", "simd2_autotune",
     "simd2_minus_square_workspace", "simd2_minus_square_exact", "simd2_minus_square_tiles",
+    "simd2_minus_square_exact_noalloc",
     "simd2_knn_select_workspace_size", "simd2_knn_select", "knn_mxu", "simd2_knn_bf16_workspace_size", "simd2_knn_bf16",
     "simd2_batch_create", "simd2_batch_destroy", "simd2_batch_upload", "simd2_batch_download",
     "simd2_batch_buffer", "simd2_batch_autotune", "simd2_batch_solve",

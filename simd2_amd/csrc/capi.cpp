@@ -245,6 +245,21 @@ int simd2_minus_square_exact(int M, int N, int K, const float *A, int lda, const
                                               workspace_bytes, stream));
 }
 
+int simd2_minus_square_exact_noalloc(int M, int N, int K, const float *A, int lda,
+                                     const float *B, int ldb, const float *C, int ldc, float *D,
+                                     int ldd, int do_epilogue, hipStream_t stream) {
+  static_assert(SIMD2_MSQ_NOALLOC_MAX_BYTES == MSQ_NOALLOC_BYTES, "scratch size");
+  // The size first: a problem past the scratch is refused before anything else.
+  if (simd2_minus_square_workspace(M, N, K) > SIMD2_MSQ_NOALLOC_MAX_BYTES)
+    return SIMD2_ERROR_WORKSPACE;
+  const bool epi = do_epilogue != 0;
+  const int st = check_cm(M, N, K, A, lda, B, ldb, C, ldc, D, ldd, epi, sizeof(float));
+  if (st == OK_NOTHING_TO_DO) return SIMD2_SUCCESS;
+  if (st != SIMD2_SUCCESS) return st;
+  return hip_status(minus_square_exact_noalloc(N, M, K, B, ldb, A, lda, D, ldd,
+                                               epi ? C : nullptr, ldc, stream));
+}
+
 int simd2_minus_square_tiles(const void *workspace, int M, int N, int K, hipStream_t stream,
                              int *mfma_tiles, int *valu_tiles) {
   if (!mfma_tiles || !valu_tiles) return SIMD2_ERROR_NULL_POINTER;

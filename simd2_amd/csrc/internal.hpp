@@ -52,6 +52,13 @@ hipError_t launch_minus_square_exact(int R, int Cn, int K, const float *X, int64
                                      const float *Y, int64_t ldy, float *O, int64_t ldo,
                                      const float *C, int64_t ldc, void *ws, size_t ws_bytes,
                                      hipStream_t s);
+// The same on the library's own scratch (a device array of MSQ_NOALLOC_BYTES
+// per device, no allocation); the caller has checked that the problem's
+// workspace fits.  Calls that overlap on one device race on the scratch.
+constexpr size_t MSQ_NOALLOC_BYTES = (size_t)40 << 20;
+hipError_t minus_square_exact_noalloc(int R, int Cn, int K, const float *X, int64_t ldx,
+                                      const float *Y, int64_t ldy, float *O, int64_t ldo,
+                                      const float *C, int64_t ldc, hipStream_t s);
 // 64x64 tiles of the last launch_minus_square_exact on this workspace and
 // problem computed on the matrix cores / on the VALU kernel; synchronises s.
 hipError_t minus_square_exact_tile_counts(const void *ws, int R, int Cn, int K, hipStream_t s,

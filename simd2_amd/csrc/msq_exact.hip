@@ -26,9 +26,16 @@
 //      (srgemm_valu.hip; every other workgroup returns at once).
 #include <stdint.h>
 
+#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
+
+// Scratch of the allocation-free entry point (minus_square_exact_noalloc):
+// the same workspace layout, held by the library on every device it runs on
+// until the process ends.  Host code never uses this name as an address; it
+// takes the device address from hipGetSymbolAddress, per device.
+static __device__ __align__(256) unsigned char scratch[simd2::MSQ_NOALLOC_BYTES];
 
 namespace simd2 {
 
@@ -406,6 +413,40 @@ hipError_t launch_minus_square_exact(int R, int Cn, int K, const float *X, int64
   ep.gate = gate;
   ep.gate_ld = w.gate_ld;
   return launch_valu_f32(8, R, Cn, K, X, ldx, Y, ldy, O, ldo, ep, s, CFG_S64);
+}
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+void *scratch_ptr[MAX_DEVICES];  // device address of `scratch`, per device
+}  // namespace
+
+hipError_t minus_square_exact_noalloc(int R, int Cn, int K, const float *X, int64_t ldx,
+                                      const float *Y, int64_t ldy, float *O, int64_t ldo,
+                                      const float *C, int64_t ldc, hipStream_t s) {
+  const size_t need = minus_square_exact_workspace(R, Cn, K);
+  if (need > MSQ_NOALLOC_BYTES) return hipErrorInvalidValue;  // the caller checks first
+  void *ws = nullptr;
+  if (need > 0) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+      // First call on this device (it is current): its copy of the array.
+      void *p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) return e;
+      if (!p) return hipErrorInvalidSymbol;
+      scratch_ptr[dev] = p;
+    }
+    ws = scratch_ptr[dev];
+  }
+  // Every byte the kernels read is written earlier in the same call
+  // (quantized operands with their padding, norms, one gate word per tile),
+  // so the previous call's contents need no clearing.
+  return launch_minus_square_exact(R, Cn, K, X, ldx, Y, ldy, O, ldo, C, ldc, ws, need, s);
 }
 
 hipError_t minus_square_exact_tile_counts(const void *ws, int R, int Cn, int K, hipStream_t s,
