@@ -11,6 +11,10 @@
 //   max_mul    2 x v_mul_f32 + v_max3_f32   (max_mul / min_mul)
 //   max_min    2 x v_min_f32 + v_max3_f32   (max_min / min_max)
 //   sub_fmac   v_sub_f32 + v_fmac_f32       (direct minus_square)
+//   min_i32    v_min_i32
+//   min3_i32   v_min3_i32
+//   max_min_i32  2 x v_min_i32 + v_max3_i32 (max_min / min_max on order-preserving
+//              integer keys of the floats)
 //
 // One launch per (mix, W) of 4 W waves per CU (one or two workgroups per CU).
 // Every wave reads the shader clock (s_memtime) around its loop and its
@@ -34,9 +38,10 @@ namespace {
 constexpr int ITERS = 2048;
 constexpr int CHAINS = 8;
 
-enum Mix { ADD, MIN3, MIN_PLUS, MAX_MUL, MAX_MIN, SUB_FMAC, NMIX };
-const char *kMixName[NMIX] = {"add", "min3", "min_plus", "max_mul", "max_min", "sub_fmac"};
-const int kInstrPerChainIter[NMIX] = {1, 1, 3, 3, 3, 2};
+enum Mix { ADD, MIN3, MIN_PLUS, MAX_MUL, MAX_MIN, SUB_FMAC, MIN_I32, MIN3_I32, MAX_MIN_I32, NMIX };
+const char *kMixName[NMIX] = {"add",      "min3",    "min_plus", "max_mul",    "max_min",
+                              "sub_fmac", "min_i32", "min3_i32", "max_min_i32"};
+const int kInstrPerChainIter[NMIX] = {1, 1, 3, 3, 3, 2, 1, 1, 3};
 
 // One loop iteration = one asm block over the 8 chains (the compiler puts an
 // s_nop after every asm statement, so one block per iteration keeps that to
@@ -55,6 +60,11 @@ const int kInstrPerChainIter[NMIX] = {1, 1, 3, 3, 3, 2};
 #define S2_MAX_MIN(c)                                                                       \
   "v_min_f32 %[t0], %[y" #c "], %[a]\n\tv_min_f32 %[t1], %[y" #c "], %[b]\n\t"            \
   "v_max3_f32 %[m" #c "], %[m" #c "], %[t0], %[t1]\n\t"
+#define S2_MIN_I32(c) "v_min_i32 %[m" #c "], %[m" #c "], %[a]\n\t"
+#define S2_MIN3_I32(c) "v_min3_i32 %[m" #c "], %[m" #c "], %[a], %[b]\n\t"
+#define S2_MAX_MIN_I32(c)                                                                   \
+  "v_min_i32 %[t0], %[y" #c "], %[a]\n\tv_min_i32 %[t1], %[y" #c "], %[b]\n\t"            \
+  "v_max3_i32 %[m" #c "], %[m" #c "], %[t0], %[t1]\n\t"
 #define S2_SUB_FMAC(c)                                                                      \
   "v_sub_f32 %[t0], %[y" #c "], %[a]\n\tv_fmac_f32 %[m" #c "], %[t0], %[t0]\n\t"
 #define S2_OUT_M                                                                            \
@@ -88,8 +98,14 @@ __global__ void issue_kernel(float seed, float *sink, unsigned long long *t, int
       asm volatile(S2_CH(S2_MAX_MUL) : S2_OUT_M : S2_IN_Y);
     else if constexpr (MIX == MAX_MIN)
       asm volatile(S2_CH(S2_MAX_MIN) : S2_OUT_M : S2_IN_Y);
-    else
+    else if constexpr (MIX == SUB_FMAC)
       asm volatile(S2_CH(S2_SUB_FMAC) : S2_OUT_M : S2_IN_Y);
+    else if constexpr (MIX == MIN_I32)
+      asm volatile(S2_CH(S2_MIN_I32) : S2_OUT_M : S2_IN_Y);
+    else if constexpr (MIX == MIN3_I32)
+      asm volatile(S2_CH(S2_MIN3_I32) : S2_OUT_M : S2_IN_Y);
+    else
+      asm volatile(S2_CH(S2_MAX_MIN_I32) : S2_OUT_M : S2_IN_Y);
   }
   const unsigned long long t1c = __builtin_amdgcn_s_memtime();
   float s = t0 + t1;
@@ -190,7 +206,10 @@ int main(int argc, char **argv) {
         !run<MIN_PLUS>(cus, wps, p.clockRate, json) ||
         !run<MAX_MUL>(cus, wps, p.clockRate, json) ||
         !run<MAX_MIN>(cus, wps, p.clockRate, json) ||
-        !run<SUB_FMAC>(cus, wps, p.clockRate, json))
+        !run<SUB_FMAC>(cus, wps, p.clockRate, json) ||
+        !run<MIN_I32>(cus, wps, p.clockRate, json) ||
+        !run<MIN3_I32>(cus, wps, p.clockRate, json) ||
+        !run<MAX_MIN_I32>(cus, wps, p.clockRate, json))
       return 1;
   }
   return 0;

@@ -13,7 +13,7 @@
 //     error scales with -- near the data's own spread even for data far from
 //     the origin;
 //  2. prep: centred points split into BF16 high and low parts (x - hi), dims
-//     zero-padded to a multiple of 32, and their FP32 squared norms; the
+//     zero-padded to a multiple of 32, and their FP32 squared norms; then the
 //     largest reference norm;
 //  3. pass 1: per (query, 64-reference tile) the smallest approximate
 //     distance.  q.r ~ qh.rh + qh.rl + ql.rh, three MFMAs per 32-dim step,
@@ -72,7 +72,6 @@ constexpr int CK_MARGIN = 16;
 constexpr int KMAX = 256;    // largest k of the precision form
 constexpr int CAND_CAP = 1024;
 constexpr int FB_GRID = 64;  // workgroups of the overflow pass
-constexpr int MEAN_ROWS = 64;
 
 // Minimum over the 16 lanes of a DPP row (lanes 16j .. 16j+15), in every lane
 // of the row: rotations by 8, 4, 2 and 1 within the row, each one VALU min
@@ -86,39 +85,66 @@ __device__ __forceinline__ float row_min16(float m) {
 }
 
 // --- 1. centre --------------------------------------------------------------
-// partial[b][j] = sum of ref[r][j] over rows r of block b (double, row order).
+// partial[split][j] = sum of ref[r][j] over the split's rows, in double: a
+// workgroup takes 64 dims (lane = dim) and one of MEAN_SPLITS row ranges;
+// wave w sums the w-th quarter of the range in row order, and the four wave
+// sums are added in wave order (a fixed order, whatever the scheduling).
+constexpr int MEAN_SPLITS = 64;
 __global__ __launch_bounds__(256) void mean_partial_kernel(const float *__restrict__ P,
                                                            int64_t ldp, int n, int d,
                                                            double *partial) {
-  const int r0 = bid_x() * MEAN_ROWS, r1 = min(n, r0 + MEAN_ROWS);
-  for (int j = tid_x(); j < d; j += 256) {
-    double s = 0.0;
-    for (int r = r0; r < r1; ++r) s += (double)P[(int64_t)r * ldp + j];
-    partial[(int64_t)bid_x() * d + j] = s;
-  }
+  __shared__ double ws[4][64];
+  const int lane = tid_x() % 64, wave = tid_x() / 64;
+  const int j = (bid_x() / MEAN_SPLITS) * 64 + lane;
+  const int split = bid_x() % MEAN_SPLITS;
+  const int per = (n + MEAN_SPLITS - 1) / MEAN_SPLITS;
+  const int r0 = split * per, r1 = max(r0, min(n, r0 + per));
+  const int q = (r1 - r0 + 3) / 4;
+  const int a = r0 + wave * q, b = min(r1, a + q);
+  double s = 0.0;
+  if (j < d)
+    for (int r = a; r < b; ++r) s += (double)P[(int64_t)r * ldp + j];
+  ws[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && j < d)
+    partial[(int64_t)split * d + j] = ((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane];
 }
 
-// mu[j] = (sum of the partials in block order) / n; clears the norm maximum.
+// mu[j] = (sum of the partials in split order) / n.
 __global__ __launch_bounds__(256) void mean_final_kernel(const double *__restrict__ partial,
-                                                         int nblk, int d, int n, float *mu,
-                                                         unsigned *rnmax) {
-  for (int j = bid_x() * 256 + tid_x(); j < d; j += gridDim.x * 256) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * d + j];
-    mu[j] = (float)(s / n);
+                                                         int d, int n, float *mu) {
+  const int j = bid_x() * 256 + tid_x();
+  if (j >= d) return;
+  double s = 0.0;
+  for (int b = 0; b < MEAN_SPLITS; ++b) s += partial[(int64_t)b * d + j];
+  mu[j] = (float)(s / n);
+}
+
+// rnmax = the largest reference norm (one workgroup; norms are >= 0).
+__global__ __launch_bounds__(1024) void norm_max_kernel(const float *__restrict__ rn, int n,
+                                                        float *rnmax) {
+  __shared__ float wm[16];
+  float m = 0.f;
+  for (int i = tid_x(); i < n; i += 1024) m = fmaxf(m, rn[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if (tid_x() % 64 == 0) wm[tid_x() / 64] = m;
+  __syncthreads();
+  if (tid_x() == 0) {
+    float r = wm[0];
+    for (int w = 1; w < 16; ++w) r = fmaxf(r, wm[w]);
+    *rnmax = r;
   }
-  if (bid_x() == 0 && tid_x() == 0) *rnmax = 0u;
 }
 
 // --- 2. prep ----------------------------------------------------------------
 // One wave per point: x - mu split into BF16 high and low parts (dims
-// zero-padded to dpad, stored [point][hi | lo]), the FP32 squared norm of
-// x - mu (wave-summed), and for the references the largest norm (its bits:
-// norms are >= 0, so integer order is float order).
+// zero-padded to dpad, stored [point][hi | lo]) and the FP32 squared norm of
+// x - mu (wave-summed).
 __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, int64_t ldp,
                                                    int n, int d, int dpad,
                                                    const float *__restrict__ mu, __bf16 *Pb,
-                                                   float *norm, unsigned *nmax) {
+                                                   float *norm) {
   const int lane = tid_x() % 64;
   const int p = bid_x() * 4 + tid_x() / 64;
   if (p >= n) return;
@@ -132,10 +158,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const float *__restrict__ P, 
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if (lane == 0) {
-    norm[p] = s;
-    if (nmax) atomicMax(nmax, __float_as_uint(s));
-  }
+  if (lane == 0) norm[p] = s;
 }
 
 // --- 3 / 5. the distance passes ----------------------------------------------
@@ -483,7 +506,7 @@ __device__ __forceinline__ float fkey_inv(uint32_t k) {
 __global__ __launch_bounds__(256) void knn_tau_kernel(const float *__restrict__ tmin, int ntiles,
                                                       int Q, int ck, float e_coef,
                                                       const float *__restrict__ qn,
-                                                      const unsigned *__restrict__ rnmax,
+                                                      const float *__restrict__ rnmax,
                                                       float *thr, int *cand_n, int *cand_exact,
                                                       int *ovf_n, int *big_n) {
   const int lane = tid_x() % 64;
@@ -522,7 +545,7 @@ __global__ __launch_bounds__(256) void knn_tau_kernel(const float *__restrict__ 
       else
         lo = mid + 1;
     }
-    const float eq = e_coef * (qn[q] + __uint_as_float(*rnmax));
+    const float eq = e_coef * (qn[q] + *rnmax);
     res = fkey_inv(lo) + 2.0f * eq;
     res = res + fabsf(res) * 1e-6f;  // the sums above round by less than this
   }
@@ -933,7 +956,7 @@ __global__ __launch_bounds__(64) void rerank_big_kernel(const float *__restrict_
 int dpad_of(int d) { return ((d + 31) / 32) * 32; }
 
 struct Bf16Plan {
-  int dpad, ntiles, slices, slice, ck, nblk;
+  int dpad, ntiles, slices, slice, ck;
   bool stream;
   size_t bQb, bRb, bqn, brn, bmu, bpart, bmax, btmin, bthr, bcn, bex, bcd, bci, bovf, bbig,
       bscr;
@@ -951,7 +974,6 @@ Bf16Plan bf16_plan(int Q, int R, int d, int k) {
   p.stream = p.dpad > 160;
   p.ntiles = (R + CR - 1) / CR;
   p.ck = k + CK_MARGIN;
-  p.nblk = (R + MEAN_ROWS - 1) / MEAN_ROWS;
   const int step = p.stream ? SREF : CR;
   const int steps = (R + step - 1) / step;
   const int qblocks = (Q + CQ - 1) / CQ;
@@ -967,7 +989,7 @@ Bf16Plan bf16_plan(int Q, int R, int d, int k) {
   p.bqn = a256((size_t)Q * 4);
   p.brn = a256((size_t)R * 4);
   p.bmu = a256((size_t)d * 4);
-  p.bpart = a256((size_t)p.nblk * d * 8);
+  p.bpart = a256((size_t)MEAN_SPLITS * d * 8);
   p.bmax = 256;
   p.btmin = a256((size_t)Q * p.ntiles * 4);
   p.bthr = a256((size_t)Q * 4);
@@ -1003,7 +1025,7 @@ hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, in
   float *rn = reinterpret_cast<float *>(take(p.brn));
   float *mu = reinterpret_cast<float *>(take(p.bmu));
   double *part = reinterpret_cast<double *>(take(p.bpart));
-  unsigned *rnmax = reinterpret_cast<unsigned *>(take(p.bmax));
+  float *rnmax = reinterpret_cast<float *>(take(p.bmax));
   float *tmin = reinterpret_cast<float *>(take(p.btmin));
   float *thr = reinterpret_cast<float *>(take(p.bthr));
   int *cn = reinterpret_cast<int *>(take(p.bcn));
@@ -1014,10 +1036,11 @@ hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, in
   int *big = reinterpret_cast<int *>(take(p.bbig));
   float *scr = reinterpret_cast<float *>(take(p.bscr));
 
-  mean_partial_kernel<<<p.nblk, 256, 0, s>>>(ref, ldr, R, d, part);
-  mean_final_kernel<<<(d + 255) / 256, 256, 0, s>>>(part, p.nblk, d, R, mu, rnmax);
-  prep_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, Q, d, p.dpad, mu, Qb, qn, nullptr);
-  prep_kernel<<<(R + 3) / 4, 256, 0, s>>>(ref, ldr, R, d, p.dpad, mu, Rb, rn, rnmax);
+  mean_partial_kernel<<<((d + 63) / 64) * MEAN_SPLITS, 256, 0, s>>>(ref, ldr, R, d, part);
+  mean_final_kernel<<<(d + 255) / 256, 256, 0, s>>>(part, d, R, mu);
+  prep_kernel<<<(Q + 3) / 4, 256, 0, s>>>(query, ldq, Q, d, p.dpad, mu, Qb, qn);
+  prep_kernel<<<(R + 3) / 4, 256, 0, s>>>(ref, ldr, R, d, p.dpad, mu, Rb, rn);
+  norm_max_kernel<<<1, 1024, 0, s>>>(rn, R, rnmax);
   const PassArgs a{qn, rn, thr, tmin, p.ntiles, cd, ci, cn, ovf, ovf + 1};
   const dim3 grid(((Q + CQ - 1) / CQ) * p.slices);
   // E_q = e_coef (||q - mu||^2 + max ||r - mu||^2): see the bound at the top

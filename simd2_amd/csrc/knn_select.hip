@@ -67,19 +67,21 @@ __device__ __forceinline__ void load_group(const float *__restrict__ dist, int64
   }
 }
 
-// Pass 1.  Wave w of the grid: query block w / tasks (64 queries), groups
-// (w % tasks) * GCH .. + GCH.  Writes gkey[(q - q0) * ngroups + g].
+// Pass 1.  One wave = one query block (64 queries, one per lane) x GCH groups
+// (task); writes gkey[(q - q0) * ngroups + g].
 __global__ __launch_bounds__(256) void sel_group_min_kernel(
     const float *__restrict__ dist, int64_t Q, int R, int q0, int qn, int tasks, int ngroups,
     uint64_t *__restrict__ gkey) {
   const int lane = tid_x() % 64;
-  const int w = bid_x() * 4 + tid_x() / 64;
+  // workgroup b: task b % tasks for query blocks 4 (b / tasks) .. + 3, one per
+  // wave, so the workgroup reads 1 KB of each row at a time
   const int qblocks = (qn + 63) / 64;
-  if (w >= qblocks * tasks) return;
-  const int ql = (w / tasks) * 64 + lane;  // query within the batch
+  const int qb = (bid_x() / tasks) * 4 + tid_x() / 64;
+  if (qb >= qblocks) return;
+  const int ql = qb * 64 + lane;  // query within the batch
   const bool live = ql < qn;
   const int q = q0 + (live ? ql : qn - 1);
-  const int g0 = (w % tasks) * GCH;
+  const int g0 = (bid_x() % tasks) * GCH;
   const int g1 = min(ngroups, g0 + GCH);
   uint64_t out[GCH];
   float va[SG], vb[SG];
@@ -161,7 +163,24 @@ __global__ __launch_bounds__(256) void sel_tau_kernel(const uint64_t *__restrict
   }
   uint64_t res = KEY_MAX;
   if (ngroups >= k) {
-    uint32_t lo = 0u, hi = 0xffffffffu;  // smallest distance key with >= k minima at or under it
+    // the smallest distance key with >= k minima at or under it, between the
+    // smallest and the largest of the query's minima
+    uint32_t lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+    for (int i = 0; i < TP; ++i)
+      if (lane + 64 * i < ngroups) {
+        lo = min(lo, (uint32_t)(kv[i] >> 32));
+        hi = max(hi, (uint32_t)(kv[i] >> 32));
+      }
+    for (int j = lane + 64 * TP; j < ngroups; j += 64) {
+      lo = min(lo, (uint32_t)(t[j] >> 32));
+      hi = max(hi, (uint32_t)(t[j] >> 32));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = min(lo, (uint32_t)__shfl_xor((int)lo, off));
+      hi = max(hi, (uint32_t)__shfl_xor((int)hi, off));
+    }
     while (lo < hi) {
       const uint32_t mid = lo + (hi - lo) / 2;
       const int c = wave_count<TP>(kv, t, ngroups, lane,
@@ -175,7 +194,10 @@ __global__ __launch_bounds__(256) void sel_tau_kernel(const uint64_t *__restrict
     const int below = wave_count<TP>(kv, t, ngroups, lane,
                                      [&](uint64_t x) { return (uint32_t)(x >> 32) < th; });
     const int need = k - below;  // >= 1: the need-th smallest index among minima with distance th
+    const int ties = wave_count<TP>(kv, t, ngroups, lane,
+                                    [&](uint64_t x) { return (uint32_t)(x >> 32) == th; });
     uint32_t ilo = 0u, ihi = 0xffffffffu;
+    if (need == ties) ilo = ihi = 0xffffffffu;  // all of them: any index bound at or above theirs
     while (ilo < ihi) {
       const uint32_t mid = ilo + (ihi - ilo) / 2;
       const int c = wave_count<TP>(kv, t, ngroups, lane, [&](uint64_t x) {
@@ -200,15 +222,15 @@ __global__ __launch_bounds__(256) void sel_collect_kernel(
     const float *__restrict__ dist, int64_t Q, int R, int q0, int qn, int tasks, int ngroups,
     const uint64_t *__restrict__ tau, uint64_t *__restrict__ cand, int cap, int *cnt) {
   const int lane = tid_x() % 64;
-  const int w = bid_x() * 4 + tid_x() / 64;
   const int qblocks = (qn + 63) / 64;
-  if (w >= qblocks * tasks) return;
-  const int ql = (w / tasks) * 64 + lane;
+  const int qb = (bid_x() / tasks) * 4 + tid_x() / 64;
+  if (qb >= qblocks) return;
+  const int ql = qb * 64 + lane;
   const bool live = ql < qn;
   const int q = q0 + (live ? ql : qn - 1);
   const uint64_t tq = live ? tau[ql] : 0ull;
   const uint32_t th = (uint32_t)(tq >> 32), tl = (uint32_t)tq;
-  const int g0 = (w % tasks) * GCH;
+  const int g0 = (bid_x() % tasks) * GCH;
   const int g1 = min(ngroups, g0 + GCH);
   auto scan = [&](const float (&v)[SG], int g) {
     const int rb = g * SG;
@@ -391,12 +413,11 @@ hipError_t launch_knn_select(const float *dist, int Q, int R, int k, float *out_
   int *big_list = big_n + 1;
   for (int q0 = 0; q0 < Q; q0 += p.qb) {
     const int qn = std::min(p.qb, Q - q0);
-    const int waves = ((qn + 63) / 64) * p.tasks;
-    sel_group_min_kernel<<<(waves + 3) / 4, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups,
-                                                         gkey);
+    const int wgs = ((qn + 255) / 256) * p.tasks;  // 4 query blocks x one task each
+    sel_group_min_kernel<<<wgs, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups, gkey);
     sel_tau_kernel<<<(qn + 3) / 4, 256, 0, s>>>(gkey, p.ngroups, qn, k, tau, cnt, big_n);
-    sel_collect_kernel<<<(waves + 3) / 4, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups,
-                                                       tau, cand, p.cap, cnt);
+    sel_collect_kernel<<<wgs, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups, tau, cand,
+                                           p.cap, cnt);
     sel_final_small_kernel<<<(qn + 3) / 4, 256, 0, s>>>(cand, cnt, p.cap, q0, qn, Q, k, out_d,
                                                         out_i, do_sqrt, big_n, big_list);
     sel_final_big_kernel<<<BIG_GRID, BIG_THREADS, 0, s>>>(cand, cnt, p.cap, q0, Q, k, out_d,
