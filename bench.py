@@ -389,6 +389,55 @@ def cpu_baseline(n=4096, apsp=None, samples=None):
     return out
 
 
+def off_grid(torch, simd2_amd, samples, rates, info):
+    """plus_mul (matrix cores) and min_plus (VALU) through the drop-ins on
+    shapes off every tile grid: 4000^3, and 1000 x 777 x 300 (row-major
+    M x K times K x N).  Median of 20 launches each; sampled rows go to the
+    checker under 'off_grid_<op>_<shape>'."""
+    stream = torch.cuda.current_stream().cuda_stream
+    L = simd2_amd.lib()
+    out = {}
+    for (M, N, K) in ((4000, 4000, 4000), (1000, 777, 300)):
+        for op in ("plus_mul", "min_plus"):
+            g = torch.Generator(device="cuda")
+            g.manual_seed(M + N + K)
+            A = torch.randint(1, 101, (M, K), generator=g, device="cuda").float()
+            B = torch.randint(1, 101, (K, N), generator=g, device="cuda").float()
+            C = torch.randint(1, 101, (M, N), generator=g, device="cuda").float()
+            if op == "min_plus":
+                A[torch.rand((M, K), generator=g, device="cuda") < 0.1] = float("inf")
+            D = torch.empty_like(C)
+            fn = getattr(L, simd2_amd.DROPIN[op])
+            args = (N, M, K, B.data_ptr(), N, A.data_ptr(), K, C.data_ptr(), N, D.data_ptr(),
+                    True, stream)
+
+            def call():
+                st = fn(*args)
+                if st:
+                    raise RuntimeError(f"{op} {M}x{N}x{K} status {st}")
+            call()
+            torch.cuda.synchronize()
+            ms = time_launches_median(torch, call)
+            tops = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+            key = f"{op}_{M}x{N}x{K}"
+            rec = {"ms": round(ms, 4), "GOPS": round(tops * 1e3, 1)}
+            if op == "plus_mul":
+                rec["frac_fp32_mfma_peak"] = round(tops / PEAK_FP32_VECTOR_TFLOPS, 4)
+            else:
+                ceil = issue_ceiling(op, rates, info)
+                rec["frac_fp32_vector_peak"] = round(tops / PEAK_FP32_VECTOR_TFLOPS, 4)
+                rec["frac_issue_ceiling"] = round(tops / ceil, 4) if ceil else None
+            out[key] = rec
+            rows = np.linspace(0, M - 1, SAMPLE_ROWS).astype(np.int64)
+            idx = torch.from_numpy(rows).cuda()
+            samples["off_grid_" + key] = (op, {
+                "A": A[idx].cpu().numpy(), "B": B.cpu().numpy(), "C": C[idx].cpu().numpy(),
+                "D": D[idx].cpu().numpy()})
+            del A, B, C, D
+    torch.cuda.empty_cache()
+    return out
+
+
 def apsp_1024(torch, simd2_amd, keep=None):
     """n=1024 ER p=0.05 seed 0 APSP: converge-loop wall time per solve.
     `keep` (a list) receives (X0, final matrix, squarings) on the host."""
@@ -749,6 +798,8 @@ def main():
         log(f"min_plus 8192^3: {ms:.3f} ms {tops:.2f} T ops/s")
         del c
         torch.cuda.empty_cache()
+        extras["off_grid"] = off_grid(torch, simd2_amd, samples, rates, info)
+        log(f"off-grid shapes: {extras['off_grid']}")
         extras["apsp_er1024"] = apsp_1024(torch, simd2_amd, apsp_keep)
         log(f"apsp n=1024: {extras['apsp_er1024']}")
         extras["pld_knn"] = pld_knn(torch, simd2_amd)
@@ -776,6 +827,9 @@ def main():
                 per_op[op]["parity"] = par[op]
         if "min_plus_8192" in par:
             extras["min_plus_8192"]["parity"] = par["min_plus_8192"]
+        for key in extras.get("off_grid", {}):
+            if "off_grid_" + key in par:
+                extras["off_grid"][key]["parity"] = par["off_grid_" + key]
         log(f"cpu baseline: {cpu}")
         log(f"parity: {par}")
 

@@ -427,6 +427,38 @@ int simd2_converge_i32(int n, int *X, int ldx, int *work, int ldw, int *flag_d, 
   int64_t ldcur = ldx, ldnxt = ldw;
   int itr = 0, run = 1;
   hipError_t e = hipSuccess;
+  // Fused loop, n >= 64: every squaring on bit-packed rows -- the same 0/1
+  // matrices bit for bit, 32x less to move.  The two packed copies live in
+  // work's rows (row r's first 2 ceil(n/32) <= n ints; nothing past column n
+  // is touched).  The first squaring's change test is the int compare's:
+  // its 0/1 result differs from X wherever X holds a value other than 0 or 1,
+  // which the packing reports into the same flag.  The result is unpacked
+  // into X.
+  if (flags == SIMD2_CONVERGE_FUSED && n >= 64 && max_itr > 0) {
+    const int nw = (n + 31) / 32;
+    uint32_t *pc = reinterpret_cast<uint32_t *>(work), *pn = pc + nw;
+    e = launch_fill_i32(flag_d, 0, 1, stream);
+    if (e == hipSuccess) e = launch_pack_i32(n, n, X, ldx, pc, ldw, flag_d, stream);
+    while (e == hipSuccess && run && itr < max_itr) {
+      ++itr;
+      if (itr > 1) e = launch_fill_i32(flag_d, 0, 1, stream);
+      if (e == hipSuccess)
+        e = launch_orand_packed(n, n, n, pc, ldw, pc, ldw, pc, ldw, pn, ldw, pc, ldw, flag_d,
+                                stream);
+      if (e != hipSuccess) break;
+      std::swap(pc, pn);
+      const int st = read_flag(flag_d, stream, &run);
+      if (st != SIMD2_SUCCESS) {
+        if (num_itr) *num_itr = itr;
+        return st;
+      }
+    }
+    if (e == hipSuccess) e = launch_unpack_i32(n, n, pc, ldw, X, ldx, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (num_itr) *num_itr = itr;
+    if (e != hipSuccess) return hip_status(e);
+    return run ? SIMD2_ERROR_NOT_CONVERGED : SIMD2_SUCCESS;
+  }
   while (run && itr < max_itr) {
     ++itr;
     e = launch_fill_i32(flag_d, 0, 1, stream);

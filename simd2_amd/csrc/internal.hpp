@@ -77,6 +77,12 @@ hipError_t launch_orand_i32(int R, int Cn, int K, const int *X, int64_t ldx,
                             int64_t ldp, int *flag, hipStream_t s);
 
 // or_and on bit-packed rows.
+// int32 booleans (nonzero = 1) to packed rows and back (srgemm_orand.hip);
+// the packing sets *nonbool (when not null) if X holds a value other than 0/1.
+hipError_t launch_pack_i32(int rows, int cols, const int *X, int64_t ldx, uint32_t *P, int64_t ldp,
+                           int *nonbool, hipStream_t s);
+hipError_t launch_unpack_i32(int rows, int cols, const uint32_t *P, int64_t ldp, int *X,
+                             int64_t ldx, hipStream_t s);
 hipError_t launch_orand_packed(int M, int N, int K, const uint32_t *A,
                                int64_t lda, const uint32_t *B, int64_t ldb,
                                const uint32_t *C, int64_t ldc, uint32_t *D,

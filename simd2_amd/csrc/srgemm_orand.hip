@@ -303,6 +303,61 @@ hipError_t launch_orand_packed(int M, int N, int K, const uint32_t *A,
   return hipGetLastError();
 }
 
+namespace {
+
+// int32 booleans (nonzero = 1) <-> packed rows (bit c % 32 of word c / 32).
+// Pack: one wave per 64 columns of a row, one ballot, two words; a wave that
+// saw a value other than 0 or 1 sets *nonbool once.  Both kernels stride over
+// the matrix with a bounded grid.
+__global__ __launch_bounds__(256) void pack_i32_kernel(int rows, int cols, const int *__restrict__ X,
+                                                       int64_t ldx, uint32_t *P, int64_t ldp,
+                                                       int *nonbool) {
+  const int lane = tid_x() % 64;
+  const int chunks = (cols + 63) / 64;
+  const int nw = (cols + 31) / 32;
+  const int64_t total = (int64_t)rows * chunks;
+  bool odd = false;
+  for (int64_t w = (int64_t)bid_x() * 4 + tid_x() / 64; w < total; w += (int64_t)gridDim.x * 4) {
+    const int r = (int)(w / chunks), word = (int)(w % chunks) * 2;
+    const int c = word * 32 + lane;
+    const int v = c < cols ? X[(int64_t)r * ldx + c] : 0;
+    odd |= (unsigned)v > 1u;
+    const uint64_t b = __ballot(v != 0);
+    if (lane == 0) P[(int64_t)r * ldp + word] = (uint32_t)b;
+    if (lane == 1 && word + 1 < nw) P[(int64_t)r * ldp + word + 1] = (uint32_t)(b >> 32);
+  }
+  if (nonbool && __any(odd) && lane == 0) atomicOr(nonbool, 1);
+}
+
+__global__ __launch_bounds__(256) void unpack_i32_kernel(int rows, int cols, const uint32_t *P,
+                                                         int64_t ldp, int *X, int64_t ldx) {
+  const int64_t total = (int64_t)rows * cols;
+  for (int64_t e = (int64_t)bid_x() * 256 + tid_x(); e < total; e += (int64_t)gridDim.x * 256) {
+    const int r = (int)(e / cols), c = (int)(e % cols);
+    X[(int64_t)r * ldx + c] = (int)((P[(int64_t)r * ldp + c / 32] >> (c % 32)) & 1u);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_pack_i32(int rows, int cols, const int *X, int64_t ldx, uint32_t *P, int64_t ldp,
+                           int *nonbool, hipStream_t s) {
+  const int64_t waves = (int64_t)rows * ((cols + 63) / 64);
+  if (waves <= 0) return hipSuccess;
+  const int grid = (int)std::min<int64_t>((waves + 3) / 4, 4096);
+  pack_i32_kernel<<<dim3(grid), dim3(256), 0, s>>>(rows, cols, X, ldx, P, ldp, nonbool);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_i32(int rows, int cols, const uint32_t *P, int64_t ldp, int *X,
+                             int64_t ldx, hipStream_t s) {
+  const int64_t n = (int64_t)rows * cols;
+  if (n <= 0) return hipSuccess;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  unpack_i32_kernel<<<dim3(grid), dim3(256), 0, s>>>(rows, cols, P, ldp, X, ldx);
+  return hipGetLastError();
+}
+
 hipError_t launch_orand_i32(int R, int Cn, int K, const int *X, int64_t ldx,
                             const int *Y, int64_t ldy, int *O, int64_t ldo,
                             const int *C, int64_t ldc, const int *P,

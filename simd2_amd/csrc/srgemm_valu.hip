@@ -28,7 +28,14 @@ namespace simd2 {
 
 namespace {
 
-template <class Op, int BM, int BN, int TM, int TN, int BK, int WAVES, bool CHECKED>
+// Launch forms: FAST -- the problem is a whole number of tiles and slabs and
+// every operand is 16-byte aligned row by row; SPLIT -- aligned, but the
+// shape is off the grid: tiles inside the matrix run the 128-bit path (their
+// last, partial K slab checked), the edge tiles the checked path; CHECKED --
+// per-element bounds everywhere.
+enum { FORM_FAST = 0, FORM_SPLIT = 1, FORM_CHECKED = 2 };
+
+template <class Op, int BM, int BN, int TM, int TN, int BK, int WAVES, int FORM>
 __global__ __launch_bounds__((BM / TM) * (BN / TN))
 __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
     int R, int Cn, int K, const float *__restrict__ X, int64_t ldx,
@@ -38,9 +45,14 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
   int tr, tc;
   map_tile(bid_x(), tiles_r, tiles_c, tr, tc);
   if (e.gate && e.gate[tr * e.gate_ld + tc] == 0) return;
-  valu_tile<Op, BM, BN, TM, TN, BK, CHECKED>(
-      R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tr * BM, tc * BN,
-      e.tile_flags ? &e.tile_flags[tr * tiles_c + tc] : nullptr, sm);
+  int *flag = e.tile_flags ? &e.tile_flags[tr * tiles_c + tc] : nullptr;
+  const int row0 = tr * BM, col0 = tc * BN;
+  if (FORM == FORM_FAST || (FORM == FORM_SPLIT && row0 + BM <= R && col0 + BN <= Cn))
+    valu_tile<Op, BM, BN, TM, TN, BK, false, false, FORM == FORM_SPLIT>(
+        R, Cn, K, X, ldx, Y, ldy, O, ldo, e, row0, col0, flag, sm);
+  else
+    valu_tile<Op, BM, BN, TM, TN, BK, true>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, row0, col0,
+                                            flag, sm);
 }
 
 template <class Op, class Cfg>
@@ -51,17 +63,22 @@ hipError_t launch_cfg(int R, int Cn, int K, const float *X, int64_t ldx,
   const int tiles_c = (Cn + Cfg::BN - 1) / Cfg::BN;
   const int nthreads = (Cfg::BM / Cfg::TM) * (Cfg::BN / Cfg::TN);
   auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-  bool fast = R % Cfg::BM == 0 && Cn % Cfg::BN == 0 && K % Cfg::BK == 0 &&
-              ldx % 4 == 0 && ldy % 4 == 0 && ldo % 4 == 0 && aligned(X) &&
-              aligned(Y) && aligned(O);
-  if (e.C) fast = fast && e.ldc % 4 == 0 && aligned(e.C);
-  if (e.P) fast = fast && e.ldp % 4 == 0 && aligned(e.P);
+  bool vec = ldx % 4 == 0 && ldy % 4 == 0 && ldo % 4 == 0 && aligned(X) && aligned(Y) &&
+             aligned(O);
+  if (e.C) vec = vec && e.ldc % 4 == 0 && aligned(e.C);
+  if (e.P) vec = vec && e.ldp % 4 == 0 && aligned(e.P);
+  const bool grid_fit = R % Cfg::BM == 0 && Cn % Cfg::BN == 0 && K % Cfg::BK == 0;
+  // the split form pays off once some tile lies wholly inside the matrix
+  const bool split = vec && R >= Cfg::BM && Cn >= Cfg::BN;
   dim3 grid(tiles_r * tiles_c), block(nthreads);
-  if (fast)
-    srgemm_valu_kernel<Op, Cfg::BM, Cfg::BN, Cfg::TM, Cfg::TN, Cfg::BK, Cfg::WAVES, false>
+  if (vec && grid_fit)
+    srgemm_valu_kernel<Op, Cfg::BM, Cfg::BN, Cfg::TM, Cfg::TN, Cfg::BK, Cfg::WAVES, FORM_FAST>
+        <<<grid, block, 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
+  else if (split)
+    srgemm_valu_kernel<Op, Cfg::BM, Cfg::BN, Cfg::TM, Cfg::TN, Cfg::BK, Cfg::WAVES, FORM_SPLIT>
         <<<grid, block, 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
   else
-    srgemm_valu_kernel<Op, Cfg::BM, Cfg::BN, Cfg::TM, Cfg::TN, Cfg::BK, Cfg::WAVES, true>
+    srgemm_valu_kernel<Op, Cfg::BM, Cfg::BN, Cfg::TM, Cfg::TN, Cfg::BK, Cfg::WAVES, FORM_CHECKED>
         <<<grid, block, 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
   return hipGetLastError();
 }

@@ -48,12 +48,38 @@ __device__ __forceinline__ bool differs(float o, float p, int mode) {
 // registers.  Fast path: 128-bit loads; checked path: per-element bounds
 // with the op's padding values.
 template <class Op, int BM, int BN, int BK, int NT, bool CHECKED>
+__device__ __forceinline__ void load_slab_form(vf4 (&xv)[BM * BK / 4 / NT],
+                                               vf4 (&yv)[BK * BN / 4 / NT],
+                                               const float *__restrict__ X, int64_t ldx,
+                                               const float *__restrict__ Y, int64_t ldy,
+                                               int R, int Cn, int K, int row0, int col0,
+                                               int k0, int tid);
+
+// KT: the tile lies inside the matrix but K may end inside a slab -- that
+// slab (uniform over the workgroup) takes the checked form, the others the
+// fast one.
+template <class Op, int BM, int BN, int BK, int NT, bool CHECKED, bool KT = false>
 __device__ __forceinline__ void load_slab(vf4 (&xv)[BM * BK / 4 / NT],
                                           vf4 (&yv)[BK * BN / 4 / NT],
                                           const float *__restrict__ X, int64_t ldx,
                                           const float *__restrict__ Y, int64_t ldy,
                                           int R, int Cn, int K, int row0, int col0,
                                           int k0, int tid) {
+  if (CHECKED || (KT && k0 + BK > K))
+    load_slab_form<Op, BM, BN, BK, NT, true>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, k0,
+                                             tid);
+  else
+    load_slab_form<Op, BM, BN, BK, NT, false>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, k0,
+                                              tid);
+}
+
+template <class Op, int BM, int BN, int BK, int NT, bool CHECKED>
+__device__ __forceinline__ void load_slab_form(vf4 (&xv)[BM * BK / 4 / NT],
+                                               vf4 (&yv)[BK * BN / 4 / NT],
+                                               const float *__restrict__ X, int64_t ldx,
+                                               const float *__restrict__ Y, int64_t ldy,
+                                               int R, int Cn, int K, int row0, int col0,
+                                               int k0, int tid) {
   constexpr int XL = BM * BK / 4 / NT, YL = BK * BN / 4 / NT;
 #pragma unroll
   for (int h = 0; h < XL; ++h) {
@@ -124,7 +150,8 @@ template <int BM, int BN, int BK> struct ValuSmem {
 // the K slabs, double-buffered through LDS.  LIST: only the BK-deep slabs
 // whose indices are list[0 .. nlist) (ascending, in LDS) are visited -- the
 // others are known to hold only the (+)-identity and contribute nothing.
-template <class Op, int BM, int BN, int TM, int TN, int BK, bool CHECKED, bool LIST>
+template <class Op, int BM, int BN, int TM, int TN, int BK, bool CHECKED, bool LIST,
+          bool KT = false>
 __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float *__restrict__ X,
                                               int64_t ldx, const float *__restrict__ Y,
                                               int64_t ldy, int row0, int col0,
@@ -152,7 +179,8 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
   const int nkt = LIST ? nlist : (K + BK - 1) / BK;
   if (nkt > 0) {
     const int k0 = LIST ? list[0] * BK : 0;
-    load_slab<Op, BM, BN, BK, NT, CHECKED>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, k0, tid);
+    load_slab<Op, BM, BN, BK, NT, CHECKED, KT>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, k0,
+                                               tid);
     store_slab<BM, BN, BK, NT, XLD>(Xs[0], Ys[0], xv, yv, tid);
   }
   __syncthreads();
@@ -164,8 +192,8 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
     // branch the compiler keeps the prefetch in scratch and waits for it.
     const int kn = kt + 1 < nkt ? kt + 1 : kt;
     const int knext = (LIST ? (int)list[kn] : kn) * BK;
-    load_slab<Op, BM, BN, BK, NT, CHECKED>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, knext,
-                                           tid);
+    load_slab<Op, BM, BN, BK, NT, CHECKED, KT>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0,
+                                               knext, tid);
 #pragma unroll
     for (int k = 0; k < BK; k += 2) {
       // Keep each k-pair's LDS reads next to its arithmetic: without this
@@ -210,7 +238,10 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
 //     column the tile and the K loop touch exists in memory, and the K
 //     padding holds the op's pad values -- so loads stay 128-bit; only the
 //     epilogue is limited to r < R, c < Cn.
-template <class Op, int BM, int BN, int TM, int TN, int BK, bool CHECKED, bool MASKED = false>
+//   KT (with !CHECKED): the tile lies inside the matrix and K need not be a
+//     multiple of BK: the last, partial slab is loaded checked.
+template <class Op, int BM, int BN, int TM, int TN, int BK, bool CHECKED, bool MASKED = false,
+          bool KT = false>
 __device__ __forceinline__ void valu_tile(int R, int Cn, int K, const float *__restrict__ X,
                                           int64_t ldx, const float *__restrict__ Y, int64_t ldy,
                                           float *O, int64_t ldo, const Epilogue &e, int row0,
@@ -234,8 +265,8 @@ __device__ __forceinline__ void valu_tile(int R, int Cn, int K, const float *__r
   if (tid == 0) sm.changed = 0;
 
   float acc[TM][TN];
-  valu_mainloop<Op, BM, BN, TM, TN, BK, CHECKED, false>(R, Cn, K, X, ldx, Y, ldy, row0, col0,
-                                                         nullptr, 0, sm, acc);
+  valu_mainloop<Op, BM, BN, TM, TN, BK, CHECKED, false, KT>(R, Cn, K, X, ldx, Y, ldy, row0,
+                                                             col0, nullptr, 0, sm, acc);
   (void)XL;
   (void)YL;
   (void)XLD;

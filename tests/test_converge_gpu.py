@@ -132,3 +132,29 @@ def test_comp_update_dropins(gpu):
     ci = torch.tensor([[1, 3]], dtype=torch.int32, device="cuda")
     assert simd2_amd.comp_update(pi, ci, chk)
     assert torch.equal(pi, ci)
+
+
+@pytest.mark.parametrize("n", [64, 65, 97, 300, 1000])
+def test_closure_i32_any_values_and_stride(gpu, oracle_mod, n):
+    """simd2_converge_i32 on a strided int32 matrix whose entries are not only
+    0/1 (the reference's bin_or/bin_and treat any nonzero as true): both loop
+    structures give the oracle's closure and squaring count, also when
+    max_itr stops the loop early (the matrix after max_itr squarings)."""
+    import simd2_amd
+    rng = np.random.default_rng(n)
+    vals = np.array([0, 1, 7, -2], np.int32)
+    X0 = np.where(rng.random((n, n)) < 2.0 / n, rng.choice(vals[1:], (n, n)), 0).astype(np.int32)
+    X0[rng.integers(0, n, n // 4), rng.integers(0, n, n // 4)] = -2
+    Xo, itr_o = oracle_mod.converge_i32(X0)
+    assert itr_o >= 3
+    for flags in (simd2_amd.CONVERGE_REFERENCE, simd2_amd.CONVERGE_FUSED):
+        for max_itr in (1, 2, itr_o - 1, 500):
+            say(f"closure n={n} flags={flags} max_itr={max_itr}")
+            want, it_w = (Xo, itr_o) if max_itr == 500 else oracle_mod.converge_i32(X0, max_itr)
+            wide = gpu.full((n, n + 37), 5, dtype=gpu.int32, device="cuda")
+            wide[:, :n] = gpu.from_numpy(X0).cuda()
+            X, itr, st = simd2_amd.converge("or_and", wide[:, :n], flags=flags, max_itr=max_itr)
+            assert itr == it_w, (flags, max_itr)
+            assert st == (0 if max_itr == 500 else simd2_amd.ERROR_NOT_CONVERGED)
+            assert np.array_equal(X.cpu().numpy(), want), (flags, max_itr)
+            assert bool((wide[:, n:] == 5).all()), "wrote past the row"

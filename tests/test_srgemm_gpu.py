@@ -286,6 +286,37 @@ def test_every_tile_config_identical(gpu, oracle_mod, M, N, K):
             assert np.array_equal(bits(D.cpu().numpy()), bits(want)), (op, cfg)
 
 
+@pytest.mark.parametrize("M,N,K", [(200, 132, 68), (129, 68, 20), (64, 64, 20), (300, 64, 4),
+                                   (1000, 772, 300)])
+def test_split_form_off_grid(gpu, oracle_mod, M, N, K):
+    """Off-grid shapes with 16-byte-aligned rows (K and N multiples of 4): the
+    tiles inside the matrix take the 128-bit path with a checked last K slab,
+    the edge tiles the checked path.  Every op, every tile configuration,
+    with and without C, and C and D wider than N (leading dimension N + 4,
+    D's padding untouched)."""
+    import simd2_amd
+    for op in EXACT_OPS:
+        A, B, C = semiring_inputs(op, M, N, K, seed=M + 3 * N + 7 * K)
+        tA, tB, tC = (gpu.from_numpy(x).cuda() for x in (A, B, C))
+        for epi in (False, True):
+            want = bits(oracle_mod.srgemm_f32(op, A, B, C, epi, threads=8))
+            for cfg in range(simd2_amd.lib().simd2_tile_config_count()):
+                say(f"split {op} {M}x{N}x{K} epi={epi} cfg={cfg}")
+                D = simd2_amd.srgemm(op, tA, tB, tC, epi, tile_config=cfg)
+                gpu.cuda.synchronize()
+                assert np.array_equal(bits(D.cpu().numpy()), want), (op, epi, cfg)
+        # the drop-in's C and D share one leading dimension (kernel/srgemm.cuh)
+        Cw = gpu.full((M, N + 4), -3.0, device="cuda")
+        Cw[:, :N] = tC
+        Dw = gpu.full((M, N + 4), -5.0, device="cuda")
+        assert simd2_amd.cuasr_srsgemm(op, N, M, K, tB, N, tA, K, Cw, N + 4, Dw, True) == 0
+        gpu.cuda.synchronize()
+        got = Dw.cpu().numpy()
+        assert np.array_equal(bits(got[:, :N]), bits(oracle_mod.srgemm_f32(op, A, B, C, True,
+                                                                          threads=8))), op
+        assert np.all(got[:, N:] == -5.0), op
+
+
 def test_autotune_records_winner(gpu, oracle_mod):
     import simd2_amd
     M = N = K = 512
