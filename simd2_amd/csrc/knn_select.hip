@@ -12,17 +12,23 @@
 // distances the earlier index stays ahead.  That is exactly the order of the
 // unique 64-bit key (order-preserving bits of d) << 32 | r.
 //
-// Method (exact, no per-element lists; two streaming passes over dist):
+// Method (exact, no per-element lists):
 //   1. group minima: for each query and each group of SG = 32 consecutive
 //      references, the smallest key of the group (one wave = 64 queries, one
-//      per lane, x 16 groups; every load is 64 consecutive floats);
+//      per lane, x 16 groups; every load is 64 consecutive floats), and for
+//      k <= KF also the group's second-smallest distance;
 //   2. tau[q] = the k-th smallest of the query's group minima.  The k group
 //      minima at or below tau are k distinct references, so the k-th smallest
 //      key of the row is <= tau: every key of the answer is <= tau.  Keys
 //      strictly below tau lie in the k - 1 groups whose minimum is below tau
 //      (keys are unique), so at most (k - 1) * SG + 1 keys are <= tau;
-//   3. collect: a second pass appends every key <= tau to the query's
-//      candidate list (typically k to 2k entries; the bound above sizes it);
+//   3. collect every key <= tau into the query's candidate list (typically
+//      k to 2k entries; the bound above sizes it).  k <= KF: in the same
+//      wave as tau -- a group whose minimum is <= tau and whose second
+//      distance is above tau's contributes exactly its minimum, read from
+//      step 1; only a group holding two or more keys <= tau (rare) is read
+//      again, 32 loads by the wave -- so dist is streamed once.  Larger k: a
+//      second streaming pass over dist;
 //   4. final: sort the candidates and write the first k (square-rooted on
 //      request), one wave per query for up to 64 candidates, a workgroup
 //      with an LDS or global bitonic sort for more.
@@ -44,6 +50,7 @@ constexpr int BIG_THREADS = 1024;
 constexpr int LDS_SORT = 8192;         // keys sorted in LDS by one workgroup (64 KB)
 constexpr int BIG_GRID = 512;
 constexpr uint64_t KEY_MAX = ~0ull;
+constexpr int KF = 64;                 // largest k of the one-pass collect
 
 __device__ __forceinline__ uint32_t fkey(float f) {
   const uint32_t u = __float_as_uint(f);
@@ -57,21 +64,26 @@ __device__ __forceinline__ uint64_t key_of(float d, int r) {
 }
 
 // Loads the SG distances of group g of query q (references clamped into
-// [0, R); the caller masks the ones past R).
+// [0, R); the caller masks the ones past R).  The matrix is streamed once:
+// nontemporal loads keep it from displacing the small arrays in L2 (7%
+// faster than ordinary loads at 16384 x 16384).
 __device__ __forceinline__ void load_group(const float *__restrict__ dist, int64_t Q, int R,
                                            int q, int g, float (&v)[SG]) {
 #pragma unroll
   for (int u = 0; u < SG; ++u) {
     const int r = min(g * SG + u, R - 1);
-    v[u] = dist[(int64_t)r * Q + q];
+    v[u] = __builtin_nontemporal_load(&dist[(int64_t)r * Q + q]);
   }
 }
 
 // Pass 1.  One wave = one query block (64 queries, one per lane) x GCH groups
-// (task); writes gkey[(q - q0) * ngroups + g].
+// (task); writes gkey[(q - q0) * ngroups + g] and, with SECOND, the order
+// key of the group's second-smallest distance to g2 (same index; all ones
+// for a group of one reference).
+template <bool SECOND>
 __global__ __launch_bounds__(256) void sel_group_min_kernel(
     const float *__restrict__ dist, int64_t Q, int R, int q0, int qn, int tasks, int ngroups,
-    uint64_t *__restrict__ gkey) {
+    uint64_t *__restrict__ gkey, uint32_t *__restrict__ g2) {
   const int lane = tid_x() % 64;
   // workgroup b: task b % tasks for query blocks 4 (b / tasks) .. + 3, one per
   // wave, so the workgroup reads 1 KB of each row at a time
@@ -84,43 +96,50 @@ __global__ __launch_bounds__(256) void sel_group_min_kernel(
   const int g0 = (bid_x() % tasks) * GCH;
   const int g1 = min(ngroups, g0 + GCH);
   uint64_t out[GCH];
+  uint32_t out2[GCH];
+  // smallest key of the group (strict '<': the earliest index keeps a tie)
+  // and the second-smallest distance (ties with the minimum included)
+  auto reduce = [&](const float (&v)[SG], int g, uint64_t &o, uint32_t &o2) {
+    const int rb = g * SG;
+    float m = v[0], m2 = __int_as_float(0x7f800000);
+    int mi = 0;
+    bool two = false;
+#pragma unroll
+    for (int u = 1; u < SG; ++u)
+      if (rb + u < R) {
+        two = true;
+        if (v[u] < m) {
+          m2 = m;
+          m = v[u];
+          mi = u;
+        } else if (SECOND) {
+          m2 = fminf(m2, v[u]);
+        }
+      }
+    o = key_of(m, rb + mi);
+    o2 = two ? fkey(m2) : 0xffffffffu;
+  };
   float va[SG], vb[SG];
   load_group(dist, Q, R, q, g0, va);
 #pragma unroll
   for (int gi = 0; gi < GCH; gi += 2) {
     // two register sets: the next group's loads are in flight while one is reduced
     if (g0 + gi + 1 < g1) load_group(dist, Q, R, q, g0 + gi + 1, vb);
-    if (g0 + gi < g1) {
-      const int rb = (g0 + gi) * SG;
-      float m = va[0];
-      int mi = 0;
-#pragma unroll
-      for (int u = 1; u < SG; ++u)
-        if (rb + u < R && va[u] < m) {  // strict: the earliest index keeps a tie
-          m = va[u];
-          mi = u;
-        }
-      out[gi] = key_of(m, rb + mi);
-    }
+    if (g0 + gi < g1) reduce(va, g0 + gi, out[gi], out2[gi]);
     if (g0 + gi + 2 < g1) load_group(dist, Q, R, q, g0 + gi + 2, va);
-    if (g0 + gi + 1 < g1) {
-      const int rb = (g0 + gi + 1) * SG;
-      float m = vb[0];
-      int mi = 0;
-#pragma unroll
-      for (int u = 1; u < SG; ++u)
-        if (rb + u < R && vb[u] < m) {
-          m = vb[u];
-          mi = u;
-        }
-      out[gi + 1] = key_of(m, rb + mi);
-    }
+    if (g0 + gi + 1 < g1) reduce(vb, g0 + gi + 1, out[gi + 1], out2[gi + 1]);
   }
   if (!live) return;
   uint64_t *dst = gkey + (int64_t)ql * ngroups;
 #pragma unroll
   for (int gi = 0; gi < GCH; ++gi)
     if (g0 + gi < g1) dst[g0 + gi] = out[gi];
+  if (SECOND) {
+    uint32_t *dst2 = g2 + (int64_t)ql * ngroups;
+#pragma unroll
+    for (int gi = 0; gi < GCH; ++gi)
+      if (g0 + gi < g1) dst2[g0 + gi] = out2[gi];
+  }
 }
 
 // Number of the query's group-minimum keys satisfying pred, over the wave.
@@ -141,19 +160,10 @@ __device__ __forceinline__ int wave_count(const uint64_t (&kv)[TP], const uint64
   return c;
 }
 
-// Pass 2.  One wave per query: tau = the k-th smallest group-minimum key
+// tau of one query, by its wave: the k-th smallest group-minimum key
 // (KEY_MAX when there are fewer than k groups), by bisection on the distance
-// bits and then, among minima with that distance, on the index.  Also clears
-// the query's candidate count (and, in the first wave, the count of queries
-// for the workgroup sort).
-__global__ __launch_bounds__(256) void sel_tau_kernel(const uint64_t *__restrict__ gkey,
-                                                      int ngroups, int qn, int k, uint64_t *tau,
-                                                      int *cnt, int *big_n) {
-  const int lane = tid_x() % 64;
-  const int ql = bid_x() * 4 + tid_x() / 64;
-  if (ql == 0 && lane == 0) *big_n = 0;
-  if (ql >= qn) return;
-  const uint64_t *t = gkey + (int64_t)ql * ngroups;
+// bits and then, among minima with that distance, on the index.
+__device__ __forceinline__ uint64_t wave_tau(const uint64_t *t, int ngroups, int k, int lane) {
   constexpr int TP = 8;
   uint64_t kv[TP];
 #pragma unroll
@@ -210,10 +220,76 @@ __global__ __launch_bounds__(256) void sel_tau_kernel(const uint64_t *__restrict
     }
     res = ((uint64_t)th << 32) | ilo;
   }
+  return res;
+}
+
+// Pass 2 for k > KF.  One wave per query: tau, and the query's candidate
+// count cleared (and, in the first wave, the count of queries for the
+// workgroup sort).
+__global__ __launch_bounds__(256) void sel_tau_kernel(const uint64_t *__restrict__ gkey,
+                                                      int ngroups, int qn, int k, uint64_t *tau,
+                                                      int *cnt, int *big_n) {
+  const int lane = tid_x() % 64;
+  const int ql = bid_x() * 4 + tid_x() / 64;
+  if (ql == 0 && lane == 0) *big_n = 0;
+  if (ql >= qn) return;
+  const uint64_t res = wave_tau(gkey + (int64_t)ql * ngroups, ngroups, k, lane);
   if (lane == 0) {
     tau[ql] = res;
     cnt[ql] = 0;
   }
+}
+
+// Passes 2 and 3 for k <= KF, one wave per query: tau, then the candidates.
+// A group whose minimum key is <= tau holds no other key <= tau when its
+// second distance's order key is above tau's distance: its minimum is
+// appended as read in pass 1.  Otherwise lanes 0 .. SG-1 read the group's
+// distances again and append every key <= tau.  Positions come from wave
+// ballots in group order, so the list is written without atomics.
+__global__ __launch_bounds__(256) void sel_tau_collect_kernel(
+    const float *__restrict__ dist, int64_t Q, int R, int q0, const uint64_t *__restrict__ gkey,
+    const uint32_t *__restrict__ g2, int ngroups, int qn, int k, uint64_t *__restrict__ cand,
+    int cap, int *cnt, int *big_n) {
+  const int lane = tid_x() % 64;
+  const int ql = bid_x() * 4 + tid_x() / 64;
+  if (ql == 0 && lane == 0) *big_n = 0;
+  if (ql >= qn) return;
+  const uint64_t *t = gkey + (int64_t)ql * ngroups;
+  const uint32_t *t2 = g2 + (int64_t)ql * ngroups;
+  const uint64_t res = wave_tau(t, ngroups, k, lane);
+  const uint32_t th = (uint32_t)(res >> 32);
+  const int q = q0 + ql;
+  const uint64_t below = (1ull << lane) - 1ull;  // lanes under this one
+  uint64_t *list = cand + (int64_t)ql * cap;
+  int base = 0;
+  for (int j0 = 0; j0 < ngroups; j0 += 64) {
+    const int j = j0 + lane;
+    const uint64_t key = j < ngroups ? t[j] : KEY_MAX;
+    const bool qual = j < ngroups && key <= res;
+    const bool again = qual && t2[j] <= th;
+    const uint64_t bd = __ballot(qual && !again);
+    if (qual && !again) {
+      const int pos = base + (int)__popcll(bd & below);
+      if (pos < cap) list[pos] = key;
+    }
+    base += (int)__popcll(bd);
+    uint64_t br = __ballot(again);
+    while (br) {
+      const int g = j0 + (int)__builtin_ctzll(br);
+      br &= br - 1;
+      const int r = g * SG + lane;
+      uint64_t kk = KEY_MAX;
+      if (lane < SG && r < R) kk = key_of(dist[(int64_t)r * Q + q], r);
+      const bool in = lane < SG && r < R && kk <= res;
+      const uint64_t bi = __ballot(in);
+      if (in) {
+        const int pos = base + (int)__popcll(bi & below);
+        if (pos < cap) list[pos] = kk;
+      }
+      base += (int)__popcll(bi);
+    }
+  }
+  if (lane == 0) cnt[ql] = base;
 }
 
 // Pass 3.  Same wave tasks as pass 1: every key <= tau of the query goes to
@@ -361,8 +437,9 @@ __global__ __launch_bounds__(BIG_THREADS) void sel_final_big_kernel(
 
 struct SelPlan {
   int ngroups, tasks, cap, qb;
-  size_t b_gkey, b_tau, b_cnt, b_cand, b_list;
-  size_t bytes() const { return b_gkey + b_tau + b_cnt + b_cand + b_list + 256; }
+  bool one_pass;
+  size_t b_gkey, b_g2, b_tau, b_cnt, b_cand, b_list;
+  size_t bytes() const { return b_gkey + b_g2 + b_tau + b_cnt + b_cand + b_list + 256; }
 };
 
 size_t a256(size_t b) { return (b + 255) / 256 * 256; }
@@ -379,7 +456,11 @@ SelPlan sel_plan(int Q, int R, int k) {
   int cap = 1;
   while (cap < bound) cap <<= 1;
   p.cap = cap;
-  const size_t per_q = (size_t)p.ngroups * 8 + (size_t)cap * 8 + 8 + 4 + 4;
+  // one pass when k is small and the row has enough groups that tau is a
+  // real bound (with fewer than 4k groups most of them would be re-read)
+  p.one_pass = k <= KF && p.ngroups >= 4 * k;
+  const size_t per_q =
+      (size_t)p.ngroups * (p.one_pass ? 12 : 8) + (size_t)cap * 8 + 8 + 4 + 4;
   size_t qb = SEL_BUDGET / per_q;
   qb = qb / 64 * 64;
   if (qb < 64) qb = 64;
@@ -387,6 +468,7 @@ SelPlan sel_plan(int Q, int R, int k) {
   if (qb > qpad) qb = qpad;
   p.qb = (int)qb;
   p.b_gkey = a256(qb * p.ngroups * 8);
+  p.b_g2 = p.one_pass ? a256(qb * p.ngroups * 4) : 0;
   p.b_tau = a256(qb * 8);
   p.b_cnt = a256(qb * 4);
   p.b_cand = a256(qb * (size_t)cap * 8);
@@ -406,18 +488,32 @@ hipError_t launch_knn_select(const float *dist, int Q, int R, int k, float *out_
   const SelPlan p = sel_plan(Q, R, k);
   char *w = static_cast<char *>(workspace);
   uint64_t *gkey = reinterpret_cast<uint64_t *>(w);
-  uint64_t *tau = reinterpret_cast<uint64_t *>(w + p.b_gkey);
-  int *cnt = reinterpret_cast<int *>(w + p.b_gkey + p.b_tau);
-  uint64_t *cand = reinterpret_cast<uint64_t *>(w + p.b_gkey + p.b_tau + p.b_cnt);
-  int *big_n = reinterpret_cast<int *>(w + p.b_gkey + p.b_tau + p.b_cnt + p.b_cand);
+  w += p.b_gkey;
+  uint32_t *g2 = reinterpret_cast<uint32_t *>(w);
+  w += p.b_g2;
+  uint64_t *tau = reinterpret_cast<uint64_t *>(w);
+  w += p.b_tau;
+  int *cnt = reinterpret_cast<int *>(w);
+  w += p.b_cnt;
+  uint64_t *cand = reinterpret_cast<uint64_t *>(w);
+  w += p.b_cand;
+  int *big_n = reinterpret_cast<int *>(w);
   int *big_list = big_n + 1;
   for (int q0 = 0; q0 < Q; q0 += p.qb) {
     const int qn = std::min(p.qb, Q - q0);
     const int wgs = ((qn + 255) / 256) * p.tasks;  // 4 query blocks x one task each
-    sel_group_min_kernel<<<wgs, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups, gkey);
-    sel_tau_kernel<<<(qn + 3) / 4, 256, 0, s>>>(gkey, p.ngroups, qn, k, tau, cnt, big_n);
-    sel_collect_kernel<<<wgs, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups, tau, cand,
-                                           p.cap, cnt);
+    if (p.one_pass) {
+      sel_group_min_kernel<true><<<wgs, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups,
+                                                     gkey, g2);
+      sel_tau_collect_kernel<<<(qn + 3) / 4, 256, 0, s>>>(dist, Q, R, q0, gkey, g2, p.ngroups,
+                                                          qn, k, cand, p.cap, cnt, big_n);
+    } else {
+      sel_group_min_kernel<false><<<wgs, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups,
+                                                      gkey, nullptr);
+      sel_tau_kernel<<<(qn + 3) / 4, 256, 0, s>>>(gkey, p.ngroups, qn, k, tau, cnt, big_n);
+      sel_collect_kernel<<<wgs, 256, 0, s>>>(dist, Q, R, q0, qn, p.tasks, p.ngroups, tau, cand,
+                                             p.cap, cnt);
+    }
     sel_final_small_kernel<<<(qn + 3) / 4, 256, 0, s>>>(cand, cnt, p.cap, q0, qn, Q, k, out_d,
                                                         out_i, do_sqrt, big_n, big_list);
     sel_final_big_kernel<<<BIG_GRID, BIG_THREADS, 0, s>>>(cand, cnt, p.cap, q0, Q, k, out_d,

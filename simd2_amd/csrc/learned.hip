@@ -210,6 +210,121 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
   __syncthreads();
 }
 
+// Searches of graphs with n <= SB_N: the whole packed adjacency (128 KB), the
+// frontier lists and the visited set live in LDS, so a level costs two
+// barriers and LDS latency instead of two dependent global loads.  The
+// frontier's rows are OR-ed into a level bitset by LDS atomics (thread t owns
+// word t % 32 for every eighth frontier vertex); wave 0 then takes the fresh
+// bits, numbers them by a prefix sum over the words and writes the next
+// list.  Same results as bfs(): depth, reached count, lowest vertex of the
+// last level, levels.
+constexpr int SB_NW = 32;
+constexpr int SB_N = SB_NW * 32;
+
+struct SmallBfsSmem {
+  uint32_t adj[SB_N * SB_NW];
+  int list[2][SB_N];
+  uint32_t visited[SB_NW];
+  uint32_t nextb[SB_NW];
+  int cnt, far;
+};
+
+__device__ void load_adj_small(const uint32_t *adj, int n, int nw, SmallBfsSmem &sm) {
+  for (int i = tid_x(); i < n * nw; i += BFS_T) sm.adj[i] = adj[i];
+  __syncthreads();
+}
+
+__device__ void bfs_small(int n, int nw, int src, int *level, int *res, SmallBfsSmem &sm) {
+  const int t = tid_x();
+  constexpr int G = BFS_T / SB_NW;
+  const int g = t / SB_NW, w = t % SB_NW;
+  if (t < SB_NW) {
+    sm.visited[t] = 0;
+    sm.nextb[t] = 0;
+  }
+  if (level)
+    for (int v = t; v < n; v += BFS_T) level[v] = v == src ? 0 : -1;
+  __syncthreads();
+  if (t == 0) {
+    sm.visited[src / 32] |= 1u << (src % 32);
+    sm.list[0][0] = src;
+    sm.cnt = 1;
+  }
+  __syncthreads();
+  int depth = 0, reached = 1, far = src, p = 0;
+  for (;;) {
+    const int cnt = sm.cnt;
+    const int *cur = sm.list[p];
+    uint32_t acc = 0;
+    if (w < nw)
+      for (int i = g; i < cnt; i += G) acc |= sm.adj[cur[i] * nw + w];
+    if (acc) atomicOr(&sm.nextb[w], acc);
+    __syncthreads();
+    if (t < 64) {
+      const int lane = t;
+      uint32_t fresh = 0;
+      if (lane < nw) {
+        fresh = sm.nextb[lane] & ~sm.visited[lane];
+        sm.visited[lane] |= fresh;
+        sm.nextb[lane] = 0;
+      }
+      const int c = __popc(fresh);
+      int incl = c;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+      }
+      int at = incl - c;
+      int *nxt = sm.list[p ^ 1];
+      for (uint32_t f = fresh; f; f &= f - 1) {
+        const int v = lane * 32 + __ffs(f) - 1;
+        nxt[at++] = v;
+        if (level) level[v] = depth + 1;
+      }
+      const uint64_t nz = __ballot(fresh != 0);
+      if (lane == 63) sm.cnt = incl;
+      if (nz && lane == __ffsll((unsigned long long)nz) - 1) sm.far = lane * 32 + __ffs(fresh) - 1;
+    }
+    __syncthreads();
+    const int ncnt = sm.cnt;
+    if (ncnt == 0) break;
+    ++depth;
+    reached += ncnt;
+    far = sm.far;
+    p ^= 1;
+  }
+  if (t == 0) {
+    res[0] = depth;
+    res[1] = reached;
+    res[2] = far;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(BFS_T) void bfs_sources_small_kernel(const uint32_t *adj, int n,
+                                                                  int nw, int *res) {
+  __shared__ SmallBfsSmem sm;
+  load_adj_small(adj, n, nw, sm);
+  const int src = (int)((int64_t)bid_x() * n / NSRC);
+  bfs_small(n, nw, src, nullptr, res + 3 * bid_x(), sm);
+}
+
+__global__ __launch_bounds__(BFS_T) void bfs_levels_small_kernel(const uint32_t *adj, int n,
+                                                                 int nw, int *level, int *res,
+                                                                 const int *far) {
+  __shared__ SmallBfsSmem sm;
+  load_adj_small(adj, n, nw, sm);
+  int src;
+  if (far) {
+    src = *far;
+  } else {
+    bfs_small(n, nw, 0, nullptr, res, sm);
+    src = res[2];
+  }
+  bfs_small(n, nw, src, level, res + 3, sm);
+}
+
 // Feature searches: workgroup k searches from vertex k * n / 8.
 __global__ __launch_bounds__(BFS_T) void bfs_sources_kernel(const uint32_t *adj, int n, int nw,
                                                             int *lists, int *res) {
@@ -502,7 +617,10 @@ hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int 
   e = hipGetLastError();
   if (e == hipSuccess) e = pack_adjacency(X, ld, n, adj, s);
   if (e != hipSuccess) return e;
-  bfs_sources_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, lists, res);
+  if (nw <= SB_NW)
+    bfs_sources_small_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, res);
+  else
+    bfs_sources_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, lists, res);
   const int max_based = (op == 2 || op == 3 || op == 4) ? 1 : 0;
   features_kernel<<<dim3(1), dim3(256), 0, s>>>(rstats, indeg, sym, res, n, max_based, epi, f);
   return hipGetLastError();
@@ -521,9 +639,13 @@ hipError_t launch_graph_levels(int n, const float *X, int64_t ld, int *level, vo
     hipError_t e = pack_adjacency(X, ld, n, adj, s);
     if (e != hipSuccess) return e;
   }
-  bfs_levels_kernel<<<dim3(1), dim3(BFS_T), 0, s>>>(adj, n, (n + 31) / 32, lists, level,
-                                                    res + 3 * NSRC,
-                                                    after_features ? res + 2 : nullptr);
+  const int nw = (n + 31) / 32;
+  if (nw <= SB_NW)
+    bfs_levels_small_kernel<<<dim3(1), dim3(BFS_T), 0, s>>>(adj, n, nw, level, res + 3 * NSRC,
+                                                            after_features ? res + 2 : nullptr);
+  else
+    bfs_levels_kernel<<<dim3(1), dim3(BFS_T), 0, s>>>(adj, n, nw, lists, level, res + 3 * NSRC,
+                                                      after_features ? res + 2 : nullptr);
   return hipGetLastError();
 }
 

@@ -39,9 +39,14 @@ def clustered(torch, Q, R, d, seed):
 
 
 @pytest.mark.parametrize("R,Q,k", [(300, 70, 16), (1000, 130, 32), (64, 64, 1), (257, 5, 7),
-                                   (2000, 100, 64), (2000, 67, 100), (40, 3, 40), (33, 65, 2)])
+                                   (2000, 100, 64), (2000, 67, 100), (40, 3, 40), (33, 65, 2),
+                                   (3000, 70, 16), (5000, 130, 1), (2049, 65, 8),
+                                   (8192, 33, 64)])
 def test_knn_select_vs_oracle(gpu, oracle_mod, R, Q, k):
-    """Integer distances (ties on purpose; the earlier index wins)."""
+    """Integer distances (ties on purpose; the earlier index wins).  Rows of
+    at least 4k groups of 32 take the one-pass collect (groups holding two
+    or more candidates read again; 2049 ends in a group of one reference),
+    shorter rows the second streaming pass."""
     import simd2_amd.knn as K
     rng = np.random.default_rng(R + Q + k)
     D = rng.integers(0, 50, (R, Q)).astype(np.float32)
@@ -57,7 +62,9 @@ def test_knn_select_vs_oracle(gpu, oracle_mod, R, Q, k):
                                         (10000, 3, 9000, "uniform"),    # global sort, 16384
                                         (5000, 130, 300, "decreasing"),
                                         (4096, 64, 16, "constant"),
-                                        (700, 90, 50, "mixed_inf")])
+                                        (700, 90, 50, "mixed_inf"),
+                                        (4100, 40, 32, "mixed_inf"),
+                                        (5000, 50, 16, "decreasing")])
 def test_knn_select_large_k_and_adversarial_rows(gpu, oracle_mod, R, Q, k, kind):
     """Large k (the candidate sort in LDS and in global memory), rows whose
     distances fall with the index (every element enters the reference's

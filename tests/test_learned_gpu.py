@@ -20,9 +20,9 @@ def oracle_learned():
     return learned
 
 
-def graphs(torch):
+def graphs(torch, sizes=(70, 301, 1000)):
     out = []
-    for k, n in enumerate((70, 301, 1000)):
+    for k, n in enumerate(sizes):
         out.append(("min_plus", True, er_family(torch, "min_plus", n, 4.0 / n, k)))
         out.append(("max_min", True, rgg_family(torch, "max_min", n, 9, k)))
         out.append(("min_max", False, grid_family(torch, "min_max", n, k)))
@@ -30,8 +30,10 @@ def graphs(torch):
 
 
 def test_features_and_order_match_oracle(gpu, oracle_learned):
+    """Both search forms: adjacency in LDS up to n = 1024 (32 words a row),
+    in global memory above."""
     torch = gpu
-    for op, epi, X in graphs(torch):
+    for op, epi, X in graphs(torch, (70, 301, 1000, 1024, 1025, 1500)):
         print("features", op, X.shape[0], flush=True)
         f = LG.graph_features(op, X, epi).cpu().numpy()
         fo = oracle_learned.graph_features(op, X.cpu().numpy(), epi)
