@@ -21,6 +21,7 @@ namespace {
 
 constexpr int NUM_XCD = 8;
 constexpr int BIN = 32;  // size bins of the schedule: n in (32 (b-1), 32 b]
+constexpr int BRANCHES = 4;  // parallel branches of a captured round
 
 int hip_status(hipError_t e) {
   if (e == hipSuccess) return SIMD2_SUCCESS;
@@ -215,6 +216,44 @@ hipError_t launch_round(simd2_batch &b, int r, hipStream_t s) {
   return e;
 }
 
+// Round r with its class launches as parallel branches: the class counts are
+// cleared on s, then launch i runs on branch stream i % nb after an event of
+// s, and s waits for every branch before the round's update.  Graphs of
+// different classes share no memory, so the classes may overlap: the tail of
+// one launch fills with the next one's workgroups, and in the late rounds the
+// classes whose graphs take more squarings run side by side.  Used while s
+// is being captured (the branches join the capture through the events).
+hipError_t launch_round_branched(simd2_batch &b, int r, hipStream_t s,
+                                 const std::vector<hipStream_t> &br,
+                                 std::vector<hipEvent_t> &events) {
+  const int nl = (int)b.launches.size();
+  if (nl == 0) return hipSuccess;
+  auto event = [&](hipEvent_t *ev) {
+    hipError_t e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    if (e == hipSuccess) events.push_back(*ev);
+    return e;
+  };
+  hipError_t e = launch_fill_i32(b.d_class + (int64_t)(r + 1) * nl, 0, nl, s);
+  hipEvent_t fork;
+  if (e == hipSuccess) e = event(&fork);
+  if (e == hipSuccess) e = hipEventRecord(fork, s);
+  const int nb = std::min<int>((int)br.size(), nl);
+  for (int k = 0; k < nb && e == hipSuccess; ++k) e = hipStreamWaitEvent(br[k], fork, 0);
+  for (int i = 0; i < nl && e == hipSuccess; ++i) {
+    const Launch &L = b.launches[i];
+    e = launch_batch_round(L.op, L.cfg, L.grid, b.d_items, b.d_launch + i, b.d_desc, b.pool,
+                           b.active(), b.changed(), r & 1,
+                           r > 0 ? b.d_class + (int64_t)r * nl + i : nullptr, br[i % nb]);
+  }
+  for (int k = 0; k < nb && e == hipSuccess; ++k) {
+    hipEvent_t join;
+    e = event(&join);
+    if (e == hipSuccess) e = hipEventRecord(join, br[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
+  }
+  return e;
+}
+
 // Waits for the last solve (any stream) before its tables change.
 void wait_solved(simd2_batch &b) {
   if (b.solved) (void)hipEventSynchronize(b.solved);
@@ -244,11 +283,19 @@ int install(simd2_batch &b, const std::vector<int> &cfg) {
   hipStream_t cs;
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_status(e);
-  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  // branch streams of the rounds (default priority, like every stream here)
+  std::vector<hipStream_t> br;
+  std::vector<hipEvent_t> events;
+  for (int k = 0; k < BRANCHES && e == hipSuccess; ++k) {
+    hipStream_t t;
+    e = hipStreamCreateWithFlags(&t, hipStreamNonBlocking);
+    if (e == hipSuccess) br.push_back(t);
+  }
+  if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     hipError_t le = launch_batch_reset(b.d_desc, b.active(), b.changed(), b.iters(), b.count, cs);
     for (int r = 0; r < b.max_rounds && le == hipSuccess; ++r) {
-      le = launch_round(b, r, cs);
+      le = launch_round_branched(b, r, cs, br, events);
       if (le == hipSuccess)
         le = launch_batch_update(b.d_desc, b.active(), b.changed(), b.iters(),
                                  b.d_class + (int64_t)(r + 1) * b.launches.size(), b.count, r,
@@ -260,6 +307,8 @@ int install(simd2_batch &b, const std::vector<int> &cfg) {
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0);
   (void)hipStreamDestroy(cs);
+  for (hipStream_t t : br) (void)hipStreamDestroy(t);
+  for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
   if (e != hipSuccess) {
     destroy_graph(b);
     return hip_status(e);
