@@ -32,14 +32,25 @@ __device__ __forceinline__ bool is_edge(float v, int i, int j) {
 
 // Per row i: out-degree, sum |w|, sum w^2, min |w|, max |w|, sum |i-j|,
 // max |i-j|, edges with |i-j| <= 16 (doubles, fixed-order tree reduction).
+// When adj is set, the row is also packed there (bit (j & 31) of word j / 32:
+// edge (i, j)), one ballot per 64 entries.
 __global__ __launch_bounds__(256) void row_stats_kernel(const float *X, int64_t ld, int n,
-                                                        double *stats) {
+                                                        double *stats, uint32_t *adj, int nw) {
   __shared__ double sh[RS][256];
   const int i = bid_x(), t = tid_x();
+  const int lane = t % 64;
   double deg = 0, ws = 0, wq = 0, wmin = 1e300, wmax = 0, ds = 0, dmax = 0, near = 0;
-  for (int j = t; j < n; j += 256) {
-    const float v = X[(int64_t)i * ld + j];
-    if (is_edge(v, i, j)) {
+  for (int j0 = 0; j0 < n; j0 += 256) {
+    const int j = j0 + t;
+    const float v = j < n ? X[(int64_t)i * ld + j] : 0.0f;
+    const bool edge = j < n && is_edge(v, i, j);
+    if (adj) {
+      const uint64_t m = __ballot(edge);
+      const int w0 = (j0 + (t - lane)) / 32;
+      if (lane == 0 && w0 < nw) adj[(int64_t)i * nw + w0] = (uint32_t)m;
+      if (lane == 32 && w0 + 1 < nw) adj[(int64_t)i * nw + w0 + 1] = (uint32_t)(m >> 32);
+    }
+    if (edge) {
       const double w = v, aw = fabs(w), d = abs(i - j);
       deg += 1;
       ws += aw;
@@ -78,17 +89,21 @@ __global__ __launch_bounds__(256) void in_degree_kernel(const float *X, int64_t 
   if (c) atomicAdd(indeg + j, c);
 }
 
-// Reverse-edge count over the sampled rows i = k * step: out[0] += edges
-// (i, j), out[1] += those with (j, i) an edge too.
-__global__ __launch_bounds__(256) void symmetry_kernel(const float *X, int64_t ld, int n,
-                                                       int step, int *out) {
+// Reverse-edge count over the sampled rows i = k * step, from the packed
+// adjacency: out[0] += edges (i, j), out[1] += those with (j, i) an edge too.
+// Thread t takes words t, t + 256, ... of row i and tests bit i of each
+// listed row j.
+__global__ __launch_bounds__(256) void symmetry_kernel(const uint32_t *__restrict__ adj, int n,
+                                                       int nw, int step, int *out) {
   const int i = bid_x() * step;
   if (i >= n) return;
   int e = 0, s = 0;
-  for (int j = tid_x(); j < n; j += 256) {
-    if (is_edge(X[(int64_t)i * ld + j], i, j)) {
-      ++e;
-      s += is_edge(X[(int64_t)j * ld + i], j, i) ? 1 : 0;
+  for (int w = tid_x(); w < nw; w += 256) {
+    uint32_t bits = adj[(int64_t)i * nw + w];
+    e += __popc(bits);
+    for (; bits; bits &= bits - 1) {
+      const int j = w * 32 + __ffs(bits) - 1;
+      s += (adj[(int64_t)j * nw + i / 32] >> (i % 32)) & 1u;
     }
   }
   if (e) atomicAdd(out, e);
@@ -607,15 +622,14 @@ hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int 
   hipError_t e = launch_fill_i32(indeg, 0, n, s);
   if (e == hipSuccess) e = launch_fill_i32(sym, 0, 2, s);
   if (e != hipSuccess) return e;
-  row_stats_kernel<<<dim3(n), dim3(256), 0, s>>>(X, ld, n, rstats);
+  row_stats_kernel<<<dim3(n), dim3(256), 0, s>>>(X, ld, n, rstats, adj, nw);
   // about 2048 workgroups whatever n: each thread walks `chunk` rows of one column
   const int chunk = std::max(8, (int)(((int64_t)n * n) / (256 * 2048)));
   in_degree_kernel<<<dim3((n + 255) / 256, (n + chunk - 1) / chunk), dim3(256), 0, s>>>(
       X, ld, n, chunk, indeg);
   const int step = std::max(1, n / 256);
-  symmetry_kernel<<<dim3((n + step - 1) / step), dim3(256), 0, s>>>(X, ld, n, step, sym);
+  symmetry_kernel<<<dim3((n + step - 1) / step), dim3(256), 0, s>>>(adj, n, nw, step, sym);
   e = hipGetLastError();
-  if (e == hipSuccess) e = pack_adjacency(X, ld, n, adj, s);
   if (e != hipSuccess) return e;
   if (nw <= SB_NW)
     bfs_sources_small_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, res);

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <cstdint>
 #include <numeric>
 #include <vector>
@@ -156,14 +157,17 @@ int simd2_converge_learned(int op, int n, float *X, int ldx, int do_epilogue, in
   hipError_t e = launch_graph_features(op, n, X, ldx, do_epilogue, feat, ws, stream);
   if (e == hipSuccess)
     e = launch_mlp_forward(model, feat, 1, pred, model + SIMD2_MLP_PARAMS, stream);
-  float host[SIMD2_MLP_OUT + SIMD2_FEATURES];
+  // features and predictions in one copy: pred follows feat in the workspace
+  constexpr size_t pred_at = (SIMD2_FEATURES * 4 + 255) / 256 * 256;
+  float hostbuf[pred_at / 4 + SIMD2_MLP_OUT];
   if (e == hipSuccess)
-    e = hipMemcpyAsync(host, pred, SIMD2_MLP_OUT * sizeof(float), hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(host + SIMD2_MLP_OUT, feat, SIMD2_FEATURES * sizeof(float),
+    e = hipMemcpyAsync(hostbuf, feat, pred_at + SIMD2_MLP_OUT * sizeof(float),
                        hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) return hip_status(e);
+  float host[SIMD2_MLP_OUT + SIMD2_FEATURES];
+  std::memcpy(host, hostbuf + pred_at / 4, SIMD2_MLP_OUT * sizeof(float));
+  std::memcpy(host + SIMD2_MLP_OUT, hostbuf, SIMD2_FEATURES * sizeof(float));
 
   // Schedule from the prediction.  Enqueueing a squaring too many costs a
   // launch (it returns at once); one too few costs a flag read per extra

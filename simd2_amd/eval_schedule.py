@@ -78,7 +78,19 @@ def evaluate(torch, count=200, n=8192, model=None, log=None, max_itr=64):
     from .schedule import workspace_bytes
     ws_s = torch.empty(workspace_bytes(n, -1, max_itr), dtype=torch.uint8, device="cuda")
     rows = []
-    for k, (fam, op, epi, check, param, seed) in enumerate(held_out_specs(count)):
+    specs = held_out_specs(count)
+    # one untimed call of every loop first (kernel loading, first-use set-up)
+    for fam, op, epi, check, param, seed in specs[:2]:
+        X = make_graph(torch, fam, op, n, param, seed)
+        simd2_amd.converge(op, X.clone(), epilogue=epi, check=check, max_itr=max_itr,
+                           flags=simd2_amd.CONVERGE_REFERENCE)
+        simd2_amd.converge(op, X.clone(), epilogue=epi, check=check, max_itr=max_itr)
+        converge_learned(op, X.clone(), model, epilogue=epi, check=check, max_itr=max_itr,
+                         workspace=ws_l)
+        converge_scheduled(op, X.clone(), epilogue=epi, check=check, max_itr=max_itr,
+                           workspace=ws_s)
+        del X
+    for k, (fam, op, epi, check, param, seed) in enumerate(specs):
         X = make_graph(torch, fam, op, n, param, seed)
         Y = X.clone()
         t_plain, (_, itr, st) = _timed(torch, lambda: simd2_amd.converge(
