@@ -15,6 +15,7 @@
 //   min3_i32   v_min3_i32
 //   max_min_i32  2 x v_min_i32 + v_max3_i32 (max_min / min_max on order-preserving
 //              integer keys of the floats)
+//   and_or     v_and_or_b32                 (packed or_and: 32 terms per lane)
 //
 // One launch per (mix, W) of 4 W waves per CU (one or two workgroups per CU).
 // Every wave reads the shader clock (s_memtime) around its loop and its
@@ -38,10 +39,11 @@ namespace {
 constexpr int ITERS = 2048;
 constexpr int CHAINS = 8;
 
-enum Mix { ADD, MIN3, MIN_PLUS, MAX_MUL, MAX_MIN, SUB_FMAC, MIN_I32, MIN3_I32, MAX_MIN_I32, NMIX };
-const char *kMixName[NMIX] = {"add",      "min3",    "min_plus", "max_mul",    "max_min",
-                              "sub_fmac", "min_i32", "min3_i32", "max_min_i32"};
-const int kInstrPerChainIter[NMIX] = {1, 1, 3, 3, 3, 2, 1, 1, 3};
+enum Mix { ADD, MIN3, MIN_PLUS, MAX_MUL, MAX_MIN, SUB_FMAC, MIN_I32, MIN3_I32, MAX_MIN_I32, AND_OR,
+           NMIX };
+const char *kMixName[NMIX] = {"add",      "min3",    "min_plus", "max_mul",     "max_min",
+                              "sub_fmac", "min_i32", "min3_i32", "max_min_i32", "and_or"};
+const int kInstrPerChainIter[NMIX] = {1, 1, 3, 3, 3, 2, 1, 1, 3, 1};
 
 // One loop iteration = one asm block over the 8 chains (the compiler puts an
 // s_nop after every asm statement, so one block per iteration keeps that to
@@ -65,6 +67,7 @@ const int kInstrPerChainIter[NMIX] = {1, 1, 3, 3, 3, 2, 1, 1, 3};
 #define S2_MAX_MIN_I32(c)                                                                   \
   "v_min_i32 %[t0], %[y" #c "], %[a]\n\tv_min_i32 %[t1], %[y" #c "], %[b]\n\t"            \
   "v_max3_i32 %[m" #c "], %[m" #c "], %[t0], %[t1]\n\t"
+#define S2_AND_OR(c) "v_and_or_b32 %[m" #c "], %[a], %[y" #c "], %[m" #c "]\n\t"
 #define S2_SUB_FMAC(c)                                                                      \
   "v_sub_f32 %[t0], %[y" #c "], %[a]\n\tv_fmac_f32 %[m" #c "], %[t0], %[t0]\n\t"
 #define S2_OUT_M                                                                            \
@@ -104,6 +107,8 @@ __global__ void issue_kernel(float seed, float *sink, unsigned long long *t, int
       asm volatile(S2_CH(S2_MIN_I32) : S2_OUT_M : S2_IN_Y);
     else if constexpr (MIX == MIN3_I32)
       asm volatile(S2_CH(S2_MIN3_I32) : S2_OUT_M : S2_IN_Y);
+    else if constexpr (MIX == AND_OR)
+      asm volatile(S2_CH(S2_AND_OR) : S2_OUT_M : S2_IN_Y);
     else
       asm volatile(S2_CH(S2_MAX_MIN_I32) : S2_OUT_M : S2_IN_Y);
   }
@@ -209,7 +214,8 @@ int main(int argc, char **argv) {
         !run<SUB_FMAC>(cus, wps, p.clockRate, json) ||
         !run<MIN_I32>(cus, wps, p.clockRate, json) ||
         !run<MIN3_I32>(cus, wps, p.clockRate, json) ||
-        !run<MAX_MIN_I32>(cus, wps, p.clockRate, json))
+        !run<MAX_MIN_I32>(cus, wps, p.clockRate, json) ||
+        !run<AND_OR>(cus, wps, p.clockRate, json))
       return 1;
   }
   return 0;

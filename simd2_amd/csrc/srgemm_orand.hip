@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void orand_packed_kernel(
     const uint32_t *__restrict__ B, int64_t ldb, const uint32_t *C,
     int64_t ldc, uint32_t *D, int64_t ldd, const uint32_t *P, int64_t ldp,
     int *flag, uint32_t last_mask, int tiles_w, int ksplit, int kw_per) {
-  __shared__ __attribute__((aligned(16))) uint32_t Bs[PK_BK][PK_BW];
+  __shared__ __attribute__((aligned(16))) uint32_t Bs[2][PK_BK][PK_BW];
   const int tid = tid_x();
   const int tw = tid % PK_NTW;
   const int trr = tid / PK_NTW;
@@ -66,41 +66,81 @@ __global__ __launch_bounds__(256) void orand_packed_kernel(
     for (int j = 0; j < PK_TW; ++j) acc[i][j] = 0u;
 
   const int nkw = (K + 31) / 32;
-  const int kw_end = min(nkw, (split + 1) * kw_per);
-  for (int kw = split * kw_per; kw < kw_end; ++kw) {
-    // A words of this thread's rows for k in [32 kw, 32 kw + 32).
-    uint32_t aw[PK_TR];
-    uint32_t any = 0;
+  const int kw0 = split * kw_per;
+  const int kw_end = min(nkw, kw0 + kw_per);
+  // Pipeline over the K-words of the split, one barrier per word: the A words
+  // of word kw + 1 are loaded one step ahead (their tile-wide "any bit set"
+  // is taken at the barrier), and the B slab of word kw + 1 is loaded during
+  // word kw's arithmetic when that word is needed; B slabs alternate between
+  // two LDS buffers.  A word whose A bits are zero across the tile is skipped
+  // without loading its B slab.
+  auto load_a = [&](int kw, uint32_t (&aw)[PK_TR]) {
 #pragma unroll
     for (int i = 0; i < PK_TR; ++i) {
       const int r = row0 + i;
-      aw[i] = (r < M) ? A[(int64_t)r * lda + kw] : 0u;
+      aw[i] = (r < M && kw < kw_end) ? A[(int64_t)r * lda + kw] : 0u;
       if (kw == nkw - 1 && (K & 31)) aw[i] &= (1u << (K & 31)) - 1u;
-      any |= aw[i];
     }
-    const bool tile_any = __syncthreads_or(any != 0);
-    if (!tile_any) continue;  // whole slab contributes nothing
-    // Stage B rows k in the slab, words [w0, w0 + PK_BW).
-    for (int f = tid; f < PK_BK * PK_BW; f += 256) {
+  };
+  constexpr int BPER = PK_BK * PK_BW / 256;  // B words per thread per slab
+  uint32_t bpre[BPER];
+  auto load_b = [&](int kw) {
+#pragma unroll
+    for (int h = 0; h < BPER; ++h) {
+      const int f = tid + 256 * h;
       const int kk = f / PK_BW, ww = f % PK_BW;
       const int k = kw * 32 + kk, w = w0 + ww;
-      Bs[kk][ww] = (k < K && w < NW) ? B[(int64_t)k * ldb + w] : 0u;
+      bpre[h] = (k < K && w < NW) ? B[(int64_t)k * ldb + w] : 0u;
     }
-    __syncthreads();
+  };
+  auto any_of = [](const uint32_t (&aw)[PK_TR]) {
+    uint32_t x = 0;
 #pragma unroll
-    for (int kk = 0; kk < PK_BK; ++kk) {
-      uint32_t bw[PK_TW];
+    for (int i = 0; i < PK_TR; ++i) x |= aw[i];
+    return x != 0;
+  };
+  uint32_t aw_cur[PK_TR], aw_nxt[PK_TR];
+  load_a(kw0, aw_cur);
+  bool need_cur = __syncthreads_or(any_of(aw_cur));
+  if (need_cur) load_b(kw0);
+  load_a(kw0 + 1, aw_nxt);
+  int buf = 0;
+  for (int kw = kw0; kw < kw_end; ++kw) {
+    if (need_cur) {
 #pragma unroll
-      for (int j = 0; j < PK_TW; ++j) bw[j] = Bs[kk][tw * PK_TW + j];
-#pragma unroll
-      for (int i = 0; i < PK_TR; ++i) {
-        // bit kk of the A word as an all-ones / all-zeros mask (v_bfe_i32)
-        const uint32_t m = (uint32_t)((int32_t)(aw[i] << (31 - kk)) >> 31);
-#pragma unroll
-        for (int j = 0; j < PK_TW; ++j) acc[i][j] = and_or(m, bw[j], acc[i][j]);
+      for (int h = 0; h < BPER; ++h) {
+        const int f = tid + 256 * h;
+        Bs[buf][f / PK_BW][f % PK_BW] = bpre[h];
       }
     }
-    __syncthreads();
+    // publishes Bs[buf]; the buffer written here was last read two words ago,
+    // before the previous word's barrier
+    const bool need_nxt = __syncthreads_or(any_of(aw_nxt));
+    if (need_nxt) load_b(kw + 1);
+    uint32_t aw_nn[PK_TR];
+    load_a(kw + 2, aw_nn);
+    if (need_cur) {
+#pragma unroll
+      for (int kk = 0; kk < PK_BK; ++kk) {
+        uint32_t bw[PK_TW];
+#pragma unroll
+        for (int j = 0; j < PK_TW; ++j) bw[j] = Bs[buf][kk][tw * PK_TW + j];
+#pragma unroll
+        for (int i = 0; i < PK_TR; ++i) {
+          // bit kk of the A word as an all-ones / all-zeros mask (v_bfe_i32)
+          const uint32_t m = (uint32_t)((int32_t)(aw_cur[i] << (31 - kk)) >> 31);
+#pragma unroll
+          for (int j = 0; j < PK_TW; ++j) acc[i][j] = and_or(m, bw[j], acc[i][j]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < PK_TR; ++i) {
+      aw_cur[i] = aw_nxt[i];
+      aw_nxt[i] = aw_nn[i];
+    }
+    need_cur = need_nxt;
+    buf ^= 1;
   }
 
   if (ksplit > 1) {  // partial words into D (set to C or zero beforehand)
@@ -151,7 +191,8 @@ __global__ __launch_bounds__(256) void orand_init_kernel(int M, int NW, const ui
 }
 
 // *flag = 1 when D differs from P (last word masked): the change test of a
-// K-split product.
+// K-split product.  One atomic per workgroup at most (a workgroup-wide OR
+// first): when most words changed, one per wave serialised on the flag.
 __global__ __launch_bounds__(256) void orand_diff_kernel(int M, int NW, const uint32_t *D,
                                                          int64_t ldd, const uint32_t *P,
                                                          int64_t ldp, uint32_t last_mask,
@@ -163,7 +204,7 @@ __global__ __launch_bounds__(256) void orand_diff_kernel(int M, int NW, const ui
     const uint32_t m = w == NW - 1 ? last_mask : ~0u;
     changed |= (D[(int64_t)r * ldd + w] & m) != (P[(int64_t)r * ldp + w] & m);
   }
-  if (__any(changed) && tid_x() % 64 == 0) atomicOr(flag, 1);
+  if (__syncthreads_or(changed) && tid_x() == 0) atomicOr(flag, 1);
 }
 
 // ---- int32 booleans -------------------------------------------------------
@@ -306,35 +347,53 @@ hipError_t launch_orand_packed(int M, int N, int K, const uint32_t *A,
 namespace {
 
 // int32 booleans (nonzero = 1) <-> packed rows (bit c % 32 of word c / 32).
-// Pack: one wave per 64 columns of a row, one ballot, two words; a wave that
-// saw a value other than 0 or 1 sets *nonbool once.  Both kernels stride over
-// the matrix with a bounded grid.
+// Pack: a wave takes 256 columns of a row -- four 64-column loads in flight,
+// four ballots, eight words; a wave that saw a value other than 0 or 1 sets
+// *nonbool once.  Unpack: a thread writes 4 consecutive columns.  Grids are
+// 2D (column blocks x rows, rows strided past 65535), so no index needs a
+// 64-bit division.
+constexpr int PACK_COLS = 1024;  // columns per pack workgroup (4 waves x 256)
+
 __global__ __launch_bounds__(256) void pack_i32_kernel(int rows, int cols, const int *__restrict__ X,
                                                        int64_t ldx, uint32_t *P, int64_t ldp,
                                                        int *nonbool) {
   const int lane = tid_x() % 64;
-  const int chunks = (cols + 63) / 64;
   const int nw = (cols + 31) / 32;
-  const int64_t total = (int64_t)rows * chunks;
+  const int c0 = bid_x() * PACK_COLS + (tid_x() / 64) * 256;
   bool odd = false;
-  for (int64_t w = (int64_t)bid_x() * 4 + tid_x() / 64; w < total; w += (int64_t)gridDim.x * 4) {
-    const int r = (int)(w / chunks), word = (int)(w % chunks) * 2;
-    const int c = word * 32 + lane;
-    const int v = c < cols ? X[(int64_t)r * ldx + c] : 0;
-    odd |= (unsigned)v > 1u;
-    const uint64_t b = __ballot(v != 0);
-    if (lane == 0) P[(int64_t)r * ldp + word] = (uint32_t)b;
-    if (lane == 1 && word + 1 < nw) P[(int64_t)r * ldp + word + 1] = (uint32_t)(b >> 32);
+  for (int r = bid_y(); r < rows; r += gridDim.y) {
+    const int *x = X + (int64_t)r * ldx;
+    int v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = c0 + q * 64 + lane;
+      v[q] = c < cols ? x[c] : 0;
+    }
+    uint64_t b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      odd |= (unsigned)v[q] > 1u;
+      b[q] = __ballot(v[q] != 0);
+    }
+    const int word = c0 / 32 + lane;
+    if (lane < 8 && word < nw) {
+      const uint64_t bq = (lane >> 1) == 0 ? b[0] : (lane >> 1) == 1 ? b[1] : (lane >> 1) == 2 ? b[2] : b[3];
+      P[(int64_t)r * ldp + word] = (lane & 1) ? (uint32_t)(bq >> 32) : (uint32_t)bq;
+    }
   }
   if (nonbool && __any(odd) && lane == 0) atomicOr(nonbool, 1);
 }
 
 __global__ __launch_bounds__(256) void unpack_i32_kernel(int rows, int cols, const uint32_t *P,
                                                          int64_t ldp, int *X, int64_t ldx) {
-  const int64_t total = (int64_t)rows * cols;
-  for (int64_t e = (int64_t)bid_x() * 256 + tid_x(); e < total; e += (int64_t)gridDim.x * 256) {
-    const int r = (int)(e / cols), c = (int)(e % cols);
-    X[(int64_t)r * ldx + c] = (int)((P[(int64_t)r * ldp + c / 32] >> (c % 32)) & 1u);
+  const int c = (bid_x() * 256 + tid_x()) * 4;
+  if (c >= cols) return;
+  for (int r = bid_y(); r < rows; r += gridDim.y) {
+    const uint32_t *p = P + (int64_t)r * ldp;
+    int *x = X + (int64_t)r * ldx;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c + q < cols) x[c + q] = (int)((p[(c + q) / 32] >> ((c + q) % 32)) & 1u);
   }
 }
 
@@ -342,19 +401,17 @@ __global__ __launch_bounds__(256) void unpack_i32_kernel(int rows, int cols, con
 
 hipError_t launch_pack_i32(int rows, int cols, const int *X, int64_t ldx, uint32_t *P, int64_t ldp,
                            int *nonbool, hipStream_t s) {
-  const int64_t waves = (int64_t)rows * ((cols + 63) / 64);
-  if (waves <= 0) return hipSuccess;
-  const int grid = (int)std::min<int64_t>((waves + 3) / 4, 4096);
-  pack_i32_kernel<<<dim3(grid), dim3(256), 0, s>>>(rows, cols, X, ldx, P, ldp, nonbool);
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  const dim3 grid((cols + PACK_COLS - 1) / PACK_COLS, std::min(rows, 65535));
+  pack_i32_kernel<<<grid, dim3(256), 0, s>>>(rows, cols, X, ldx, P, ldp, nonbool);
   return hipGetLastError();
 }
 
 hipError_t launch_unpack_i32(int rows, int cols, const uint32_t *P, int64_t ldp, int *X,
                              int64_t ldx, hipStream_t s) {
-  const int64_t n = (int64_t)rows * cols;
-  if (n <= 0) return hipSuccess;
-  const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
-  unpack_i32_kernel<<<dim3(grid), dim3(256), 0, s>>>(rows, cols, P, ldp, X, ldx);
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  const dim3 grid((cols + 1023) / 1024, std::min(rows, 65535));
+  unpack_i32_kernel<<<grid, dim3(256), 0, s>>>(rows, cols, P, ldp, X, ldx);
   return hipGetLastError();
 }
 
