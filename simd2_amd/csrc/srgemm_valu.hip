@@ -31,8 +31,9 @@ namespace {
 // Launch forms: FAST -- the problem is a whole number of tiles and slabs and
 // every operand is 16-byte aligned row by row; SPLIT -- aligned, but the
 // shape is off the grid: tiles inside the matrix run the 128-bit path (their
-// last, partial K slab checked), the edge tiles the checked path; CHECKED --
-// per-element bounds everywhere.
+// last, partial K slab checked), the edge tiles the guarded 128-bit path
+// (rows past R read as padding, a column group past Cn element by element,
+// stores masked); CHECKED -- per-element bounds everywhere.
 enum { FORM_FAST = 0, FORM_SPLIT = 1, FORM_CHECKED = 2 };
 
 template <class Op, int BM, int BN, int TM, int TN, int BK, int WAVES, int FORM>
@@ -50,6 +51,9 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
   if (FORM == FORM_FAST || (FORM == FORM_SPLIT && row0 + BM <= R && col0 + BN <= Cn))
     valu_tile<Op, BM, BN, TM, TN, BK, false, false, FORM == FORM_SPLIT>(
         R, Cn, K, X, ldx, Y, ldy, O, ldo, e, row0, col0, flag, sm);
+  else if (FORM == FORM_SPLIT)
+    valu_tile<Op, BM, BN, TM, TN, BK, false, true, true, true>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e,
+                                                               row0, col0, flag, sm);
   else
     valu_tile<Op, BM, BN, TM, TN, BK, true>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, row0, col0,
                                             flag, sm);

@@ -47,7 +47,7 @@ __device__ __forceinline__ bool differs(float o, float p, int mode) {
 // Load one BK-deep slab of X (BM x BK, k contiguous) and Y (BK x BN) into
 // registers.  Fast path: 128-bit loads; checked path: per-element bounds
 // with the op's padding values.
-template <class Op, int BM, int BN, int BK, int NT, bool CHECKED>
+template <class Op, int BM, int BN, int BK, int NT, bool CHECKED, bool GUARD = false>
 __device__ __forceinline__ void load_slab_form(vf4 (&xv)[BM * BK / 4 / NT],
                                                vf4 (&yv)[BK * BN / 4 / NT],
                                                const float *__restrict__ X, int64_t ldx,
@@ -55,10 +55,13 @@ __device__ __forceinline__ void load_slab_form(vf4 (&xv)[BM * BK / 4 / NT],
                                                int R, int Cn, int K, int row0, int col0,
                                                int k0, int tid);
 
-// KT: the tile lies inside the matrix but K may end inside a slab -- that
-// slab (uniform over the workgroup) takes the checked form, the others the
-// fast one.
-template <class Op, int BM, int BN, int BK, int NT, bool CHECKED, bool KT = false>
+// KT: K may end inside a slab -- that slab (uniform over the workgroup) takes
+// the checked form, the others the fast one.  GUARD (with the fast form): the
+// tile may reach past row R or column Cn; a 4-float load wholly inside the
+// matrix stays one 128-bit load, X rows past R read as padding, and a Y group
+// reaching past Cn is read element by element.
+template <class Op, int BM, int BN, int BK, int NT, bool CHECKED, bool KT = false,
+          bool GUARD = false>
 __device__ __forceinline__ void load_slab(vf4 (&xv)[BM * BK / 4 / NT],
                                           vf4 (&yv)[BK * BN / 4 / NT],
                                           const float *__restrict__ X, int64_t ldx,
@@ -69,11 +72,11 @@ __device__ __forceinline__ void load_slab(vf4 (&xv)[BM * BK / 4 / NT],
     load_slab_form<Op, BM, BN, BK, NT, true>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, k0,
                                              tid);
   else
-    load_slab_form<Op, BM, BN, BK, NT, false>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, k0,
-                                              tid);
+    load_slab_form<Op, BM, BN, BK, NT, false, GUARD>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0,
+                                                     col0, k0, tid);
 }
 
-template <class Op, int BM, int BN, int BK, int NT, bool CHECKED>
+template <class Op, int BM, int BN, int BK, int NT, bool CHECKED, bool GUARD>
 __device__ __forceinline__ void load_slab_form(vf4 (&xv)[BM * BK / 4 / NT],
                                                vf4 (&yv)[BK * BN / 4 / NT],
                                                const float *__restrict__ X, int64_t ldx,
@@ -87,7 +90,10 @@ __device__ __forceinline__ void load_slab_form(vf4 (&xv)[BM * BK / 4 / NT],
     const int r = row0 + f / (BK / 4);
     const int k = k0 + (f % (BK / 4)) * 4;
     if constexpr (!CHECKED) {
-      xv[h] = *reinterpret_cast<const vf4 *>(X + (int64_t)r * ldx + k);
+      if (GUARD && r >= R)
+        xv[h] = vf4{Op::pad_x, Op::pad_x, Op::pad_x, Op::pad_x};
+      else
+        xv[h] = *reinterpret_cast<const vf4 *>(X + (int64_t)r * ldx + k);
     } else {
       float t[4];
 #pragma unroll
@@ -102,7 +108,14 @@ __device__ __forceinline__ void load_slab_form(vf4 (&xv)[BM * BK / 4 / NT],
     const int k = k0 + f / (BN / 4);
     const int c = col0 + (f % (BN / 4)) * 4;
     if constexpr (!CHECKED) {
-      yv[h] = *reinterpret_cast<const vf4 *>(Y + (int64_t)k * ldy + c);
+      if (GUARD && c + 3 >= Cn) {
+        float t[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] = c + q < Cn ? Y[(int64_t)k * ldy + c + q] : Op::pad_y;
+        yv[h] = vf4{t[0], t[1], t[2], t[3]};
+      } else {
+        yv[h] = *reinterpret_cast<const vf4 *>(Y + (int64_t)k * ldy + c);
+      }
     } else {
       float t[4];
 #pragma unroll
@@ -151,7 +164,7 @@ template <int BM, int BN, int BK> struct ValuSmem {
 // whose indices are list[0 .. nlist) (ascending, in LDS) are visited -- the
 // others are known to hold only the (+)-identity and contribute nothing.
 template <class Op, int BM, int BN, int TM, int TN, int BK, bool CHECKED, bool LIST,
-          bool KT = false>
+          bool KT = false, bool GUARD = false>
 __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float *__restrict__ X,
                                               int64_t ldx, const float *__restrict__ Y,
                                               int64_t ldy, int row0, int col0,
@@ -179,8 +192,8 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
   const int nkt = LIST ? nlist : (K + BK - 1) / BK;
   if (nkt > 0) {
     const int k0 = LIST ? list[0] * BK : 0;
-    load_slab<Op, BM, BN, BK, NT, CHECKED, KT>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0, k0,
-                                               tid);
+    load_slab<Op, BM, BN, BK, NT, CHECKED, KT, GUARD>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0,
+                                                      col0, k0, tid);
     store_slab<BM, BN, BK, NT, XLD>(Xs[0], Ys[0], xv, yv, tid);
   }
   __syncthreads();
@@ -192,8 +205,8 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
     // branch the compiler keeps the prefetch in scratch and waits for it.
     const int kn = kt + 1 < nkt ? kt + 1 : kt;
     const int knext = (LIST ? (int)list[kn] : kn) * BK;
-    load_slab<Op, BM, BN, BK, NT, CHECKED, KT>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0, col0,
-                                               knext, tid);
+    load_slab<Op, BM, BN, BK, NT, CHECKED, KT, GUARD>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0,
+                                                      col0, knext, tid);
 #pragma unroll
     for (int k = 0; k < BK; k += 2) {
       // Keep each k-pair's LDS reads next to its arithmetic: without this
@@ -238,10 +251,14 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
 //     column the tile and the K loop touch exists in memory, and the K
 //     padding holds the op's pad values -- so loads stay 128-bit; only the
 //     epilogue is limited to r < R, c < Cn.
-//   KT (with !CHECKED): the tile lies inside the matrix and K need not be a
-//     multiple of BK: the last, partial slab is loaded checked.
+//   KT (with !CHECKED): K need not be a multiple of BK: the last, partial
+//     slab is loaded checked.
+//   GUARD (with MASKED and KT): the tile reaches past row R or column Cn of
+//     an unpadded matrix whose rows are 16-byte aligned; loads wholly inside
+//     stay 128-bit, the rest are guarded (load_slab), and the epilogue is
+//     MASKED's.
 template <class Op, int BM, int BN, int TM, int TN, int BK, bool CHECKED, bool MASKED = false,
-          bool KT = false>
+          bool KT = false, bool GUARD = false>
 __device__ __forceinline__ void valu_tile(int R, int Cn, int K, const float *__restrict__ X,
                                           int64_t ldx, const float *__restrict__ Y, int64_t ldy,
                                           float *O, int64_t ldo, const Epilogue &e, int row0,
@@ -265,8 +282,9 @@ __device__ __forceinline__ void valu_tile(int R, int Cn, int K, const float *__r
   if (tid == 0) sm.changed = 0;
 
   float acc[TM][TN];
-  valu_mainloop<Op, BM, BN, TM, TN, BK, CHECKED, false, KT>(R, Cn, K, X, ldx, Y, ldy, row0,
-                                                             col0, nullptr, 0, sm, acc);
+  valu_mainloop<Op, BM, BN, TM, TN, BK, CHECKED, false, KT, GUARD>(R, Cn, K, X, ldx, Y, ldy,
+                                                                    row0, col0, nullptr, 0, sm,
+                                                                    acc);
   (void)XL;
   (void)YL;
   (void)XLD;
