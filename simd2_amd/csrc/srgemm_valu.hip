@@ -121,9 +121,11 @@ ValuCfgInfo valu_cfg_info(int cfg) {
 // (n = 1024 .. 8192, all ops): the VALU semirings need many resident waves to
 // dual-issue, so up to ~3072^2 outputs the 64x64 tiles (7-8 waves per SIMD)
 // win; at 4096^2 the 128x64 tiles balance 2048 tiles over 1024 slots; from
-// ~6144^2 up the 128x128 tiles (fewest LDS reads per op) win, except for
-// minus_square whose sub+fma pair keeps preferring occupancy.  Shapes the
-// autotuner has timed (simd2_autotune) use the recorded winner instead.
+// ~6144^2 up the 128x128 tiles (fewest LDS reads per op) win.  minus_square
+// with a short K (the kNN distance step: 16384^2 x 64..128) keeps the 128x64
+// tiles at every larger area (1.38 against 1.52 ms for 128x128 and 1.53 for
+// 64x64 at 16384^2 x 128).  Shapes the autotuner has timed (simd2_autotune)
+// use the recorded winner instead.
 namespace {
 struct TunedEntry { int op, R, Cn, K, cfg; };
 constexpr int TUNED_MAX = 64;
@@ -154,8 +156,8 @@ int valu_pick_cfg(int op, int R, int Cn, int K) {
   const double scale = 256.0 / device_cu_count();   // rule is for 256 CUs
   const double a = area * scale;
   if (a <= 3072.0 * 3072.0 * 1.01) return CFG_S64;
-  if (op == 8) return CFG_S64;
   if (a <= 4096.0 * 4096.0 * 1.01) return CFG_M128x64;
+  if (op == 8 && K <= 128) return CFG_M128x64;
   return CFG_L128;
 }
 
