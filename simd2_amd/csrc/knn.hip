@@ -213,10 +213,13 @@ __device__ __forceinline__ void pass_epilogue(const f32x4 (&acc)[QG][RB], const 
       for (int v = 0; v < 4; ++v)
 #pragma unroll
         for (int t = 0; t < RB / 4; ++t) {  // each 64-reference tile of the step
+          // branch-free: references past the slice enter as +inf
           float m = __builtin_inff();
 #pragma unroll
-          for (int rb = 4 * t; rb < 4 * t + 4; ++rb)
-            if (valid[rb]) m = fminf(m, qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v]);
+          for (int rb = 4 * t; rb < 4 * t + 4; ++rb) {
+            const float dd = qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v];
+            m = fminf(m, valid[rb] ? dd : __builtin_inff());
+          }
           // the 16 lanes of this query (one DPP row) hold the tile's 64 references
           m = row_min16(m);
           const int tile = r0 / CR + t;
