@@ -24,7 +24,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MB = 128, NB = 128, KB = 16, LDP = 16;
 
-template <bool VEC>
+template <bool VEC, bool VEC_OUT>
 __global__ __launch_bounds__(256) void plusmul_mfma_kernel(
     int R, int Cn, int K, const float *__restrict__ X, int64_t ldx,
     const float *__restrict__ Y, int64_t ldy, float *O, int64_t ldo,
@@ -106,19 +106,39 @@ __global__ __launch_bounds__(256) void plusmul_mfma_kernel(
     __syncthreads();
   }
 
+  // Stores through LDS, 16 rows of the wave's quadrant at a time: written in
+  // the accumulator layout (one column per lane, 4 rows per 16 lanes) to the
+  // wave's own padded region and read back row-major, so C is read and O
+  // written 16 bytes per lane along rows.  The loop's last barrier has
+  // retired every read of Xs; only this wave touches its region.
+  constexpr int TLD = 68;
+  float *T = &Xs[0][0][0] + wave * 16 * TLD;
+  const int qr0 = row0 + wr * 64, qc0 = col0 + wc * 64;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = row0 + wr * 64 + i * 16 + 4 * lk + v;
-        const int c = col0 + wc * 64 + j * 16 + lr;
-        if (r >= R || c >= Cn) continue;
-        float o = acc[i][j][v];
-        if (C) o = o + C[(int64_t)r * ldc + c];
-        O[(int64_t)r * ldo + c] = o;
+      for (int v = 0; v < 4; ++v) T[(4 * lk + v) * TLD + j * 16 + lr] = acc[i][j][v];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int idx = lane + 64 * h;
+      const int rr = idx / 16, cc = (idx % 16) * 4;
+      const int r = qr0 + i * 16 + rr, c = qc0 + cc;
+      f32x4 o = *reinterpret_cast<const f32x4 *>(&T[rr * TLD + cc]);
+      if (r >= R) continue;
+      if (VEC_OUT && c + 3 < Cn) {
+        if (C) o += *reinterpret_cast<const f32x4 *>(C + (int64_t)r * ldc + c);
+        *reinterpret_cast<f32x4 *>(O + (int64_t)r * ldo + c) = o;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (c + q < Cn) O[(int64_t)r * ldo + c + q] = C ? o[q] + C[(int64_t)r * ldc + c + q] : o[q];
       }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 }  // namespace
@@ -132,12 +152,20 @@ hipError_t launch_plusmul_mfma(int R, int Cn, int K, const float *X,
                    aligned(Y);
   const int tiles_c = (Cn + NB - 1) / NB;
   const dim3 grid(((R + MB - 1) / MB) * tiles_c);
-  if (vec)
-    plusmul_mfma_kernel<true><<<grid, dim3(256), 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo, C, ldc,
-                                                         tiles_c);
+  // 16-byte stores (and C loads) when every row of O and C is 16-byte aligned
+  const bool vec_out = ldo % 4 == 0 && aligned(O) && (!C || (ldc % 4 == 0 && aligned(C)));
+  if (vec && vec_out)
+    plusmul_mfma_kernel<true, true><<<grid, dim3(256), 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo, C,
+                                                               ldc, tiles_c);
+  else if (vec)
+    plusmul_mfma_kernel<true, false><<<grid, dim3(256), 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo,
+                                                                C, ldc, tiles_c);
+  else if (vec_out)
+    plusmul_mfma_kernel<false, true><<<grid, dim3(256), 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo,
+                                                                C, ldc, tiles_c);
   else
-    plusmul_mfma_kernel<false><<<grid, dim3(256), 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo, C, ldc,
-                                                          tiles_c);
+    plusmul_mfma_kernel<false, false><<<grid, dim3(256), 0, s>>>(R, Cn, K, X, ldx, Y, ldy, O, ldo,
+                                                                 C, ldc, tiles_c);
   return hipGetLastError();
 }
 
