@@ -9,6 +9,8 @@
 //                                        weight 10), seed 7 (apps/data/graph_gen.h:33-81)
 //   options: --reps N (default 10, the reference's NUM_ITR)  --loop reference|fused
 //            --check threshold|exact  --dump FILE (final matrix, raw row-major)
+//            --warmup N (default 0: untimed solves before the timed ones, so that the
+//            first launch of each kernel -- code loading -- stays out of the mean)
 //
 // Output: the mean milliseconds per solve (first line, as the reference prints it),
 // then "iters <n>" and "checksum <x>" (utils/check_sum.h:3-18).
@@ -208,12 +210,13 @@ int main(int argc, char **argv) {
     usage();
     return 1;
   }
-  int reps = 10, loop = SIMD2_CONVERGE_REFERENCE, check = spec->check;
+  int reps = 10, warmup = 0, loop = SIMD2_CONVERGE_REFERENCE, check = spec->check;
   std::string dump, file;
   std::vector<std::string> pos;
   for (int i = 2; i < argc; ++i) {
     const std::string s = argv[i];
     if (s == "--reps" && i + 1 < argc) reps = std::max(1, std::atoi(argv[++i]));
+    else if (s == "--warmup" && i + 1 < argc) warmup = std::max(0, std::atoi(argv[++i]));
     else if (s == "--loop" && i + 1 < argc) loop = std::string(argv[++i]) == "fused" ? SIMD2_CONVERGE_FUSED : SIMD2_CONVERGE_REFERENCE;
     else if (s == "--check" && i + 1 < argc) check = std::string(argv[++i]) == "exact" ? SIMD2_CHECK_EXACT : SIMD2_CHECK_THRESHOLD;
     else if (s == "--dump" && i + 1 < argc) dump = argv[++i];
@@ -256,9 +259,10 @@ int main(int argc, char **argv) {
     for (int i = 0; i < v; ++i)
       for (int j = 0; j < v; ++j)
         t0[(size_t)i * v + j] = (i == j) || g[(size_t)i * v + j] < FLT_MAX - 100;
-    for (int r = 0; r < reps; ++r) {
+    for (int r = 0; r < warmup + reps; ++r) {
       t = t0;
-      total += solve(*spec, v, t, loop, SIMD2_CHECK_EXACT, &iters);
+      const double ms = solve(*spec, v, t, loop, SIMD2_CHECK_EXACT, &iters);
+      if (r >= warmup) total += ms;
     }
     for (int x : t) cs += x;
     if (!dump.empty()) {
@@ -267,9 +271,10 @@ int main(int argc, char **argv) {
     }
   } else {
     std::vector<float> m;
-    for (int r = 0; r < reps; ++r) {
+    for (int r = 0; r < warmup + reps; ++r) {
       m = g;
-      total += solve(*spec, v, m, loop, check, &iters);
+      const double ms = solve(*spec, v, m, loop, check, &iters);
+      if (r >= warmup) total += ms;
     }
     cs = check_sum(m);
     if (!dump.empty()) {

@@ -99,7 +99,7 @@ def run_app(app, v, p, loop, reps, seed=1, binary=APP_BIN):
     else:
         cmd += ["-f"]
         stdin = edge_list(app, v, p, seed)
-    cmd += ["--reps", str(reps), "--loop", loop]
+    cmd += ["--reps", str(reps), "--warmup", "1", "--loop", loop]
     try:
         res = subprocess.run(cmd, input=stdin, capture_output=True, text=True, check=True)
     finally:
@@ -140,7 +140,7 @@ def run_app_file(app, path, loop, reps, binary=APP_BIN):
         cmd.append(tmp)
     else:
         stdin = data
-    cmd += ["--reps", str(reps), "--loop", loop]
+    cmd += ["--reps", str(reps), "--warmup", "1", "--loop", loop]
     try:
         res = subprocess.run(cmd, input=stdin, capture_output=True, text=True, check=True)
     finally:
