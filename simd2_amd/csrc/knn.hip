@@ -67,7 +67,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int QG = 2;        // groups of 16 queries per wave
 constexpr int CQ = 64 * QG;  // queries per workgroup (4 waves)
-constexpr int CR = 64;       // references per tile (tile minima granularity)
+constexpr int CR = 64;       // references per LDS step, and per tile of the tile minima
 constexpr int CK_MARGIN = 16;
 constexpr int KMAX = 256;    // largest k of the precision form
 constexpr int CAND_CAP = 1024;
@@ -191,6 +191,11 @@ __device__ __forceinline__ void append_global(const PassArgs &a, int q, float dd
   }
 }
 
+// Pass 1 writes per (query, tile) the smallest approximate distance; a tile is
+// 64 references (PASS 1) or, where R / 64 tiles would give tau too few minima
+// to be tight, 16 (PASS 3).  PASS 2 collects the candidates.
+constexpr int tile_refs(int pass) { return pass == 3 ? 16 : CR; }
+
 // The epilogue of one workgroup step over RB blocks of 16 references
 // (r0 .. r0 + 16 RB): acc[g][rb] holds the dot products of this lane's 4
 // queries of group g with reference r0 + rb*16 + lane%16.
@@ -206,24 +211,25 @@ __device__ __forceinline__ void pass_epilogue(const f32x4 (&acc)[QG][RB], const 
     rnv[rb] = Rn[rb * 16 + lane % 16];
     valid[rb] = r0 + rb * 16 + lane % 16 < rb1;
   }
-  if constexpr (PASS == 1) {
+  if constexpr (PASS != 2) {
+    constexpr int TB = tile_refs(PASS) / 16;  // 16-reference blocks per tile
 #pragma unroll
     for (int g = 0; g < QG; ++g)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
-        for (int t = 0; t < RB / 4; ++t) {  // each 64-reference tile of the step
+        for (int t = 0; t < RB / TB; ++t) {  // each tile of the step
           // branch-free: references past the slice enter as +inf
           float m = __builtin_inff();
 #pragma unroll
-          for (int rb = 4 * t; rb < 4 * t + 4; ++rb) {
+          for (int rb = TB * t; rb < TB * t + TB; ++rb) {
             const float dd = qnv[g][v] + rnv[rb] - 2.0f * acc[g][rb][v];
             m = fminf(m, valid[rb] ? dd : __builtin_inff());
           }
-          // the 16 lanes of this query (one DPP row) hold the tile's 64 references
+          // the 16 lanes of this query (one DPP row) hold the tile's references
           m = row_min16(m);
-          const int tile = r0 / CR + t;
-          if (lane % 16 == 0 && qv[g][v] < Q && r0 + CR * t < rb1)
+          const int tile = r0 / tile_refs(PASS) + t;
+          if (lane % 16 == 0 && qv[g][v] < Q && r0 + tile_refs(PASS) * t < rb1)
             a.tmin[(int64_t)qv[g][v] * a.ntiles + tile] = m;
         }
   } else {
@@ -960,7 +966,7 @@ int dpad_of(int d) { return ((d + 31) / 32) * 32; }
 
 struct Bf16Plan {
   int dpad, ntiles, slices, slice, ck;
-  bool stream;
+  bool stream, fine;
   size_t bQb, bRb, bqn, brn, bmu, bpart, bmax, btmin, bthr, bcn, bex, bcd, bci, bovf, bbig,
       bscr;
   size_t bytes() const {
@@ -975,8 +981,11 @@ Bf16Plan bf16_plan(int Q, int R, int d, int k) {
   Bf16Plan p;
   p.dpad = dpad_of(d);
   p.stream = p.dpad > 160;
-  p.ntiles = (R + CR - 1) / CR;
   p.ck = k + CK_MARGIN;
+  // 16-reference tiles while 64-reference ones would number under 4 ck: tau
+  // is the ck-th smallest minimum, and from few tiles it lies far out
+  p.fine = (R + CR - 1) / CR < 4 * p.ck;
+  p.ntiles = (R + tile_refs(p.fine ? 3 : 1) - 1) / tile_refs(p.fine ? 3 : 1);
   const int step = p.stream ? SREF : CR;
   const int steps = (R + step - 1) / step;
   const int qblocks = (Q + CQ - 1) / CQ;
@@ -1010,7 +1019,8 @@ Bf16Plan bf16_plan(int Q, int R, int d, int k) {
 
 int knn_bf16_kmax() { return KMAX; }
 
-size_t knn_bf16_workspace(int Q, int R, int d) { return bf16_plan(Q, R, d, 1).bytes(); }
+// sized for the largest k: a larger k can only choose the finer tiles
+size_t knn_bf16_workspace(int Q, int R, int d) { return bf16_plan(Q, R, d, KMAX).bytes(); }
 
 hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, int64_t ldr, int Q,
                            int R, int d, int k, float *out_d, int *out_i, int do_sqrt,
@@ -1051,14 +1061,18 @@ hipError_t launch_knn_bf16(const float *query, int64_t ldq, const float *ref, in
                                                              ldexpf(1.0f, -24));
   for (int pass = 1; pass <= 2; ++pass) {
     if (p.stream) {
-      if (pass == 1)
+      if (pass == 1 && p.fine)
+        knn_pass_stream_kernel<3><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.dpad, p.slices, p.slice, a);
+      else if (pass == 1)
         knn_pass_stream_kernel<1><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.dpad, p.slices, p.slice, a);
       else
         knn_pass_stream_kernel<2><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.dpad, p.slices, p.slice, a);
     } else {
 #define SIMD2_KNN_PASS(D)                                                                      \
   case D:                                                                                      \
-    if (pass == 1)                                                                             \
+    if (pass == 1 && p.fine)                                                                   \
+      knn_pass_kernel<D, 3><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.slices, p.slice, a);         \
+    else if (pass == 1)                                                                        \
       knn_pass_kernel<D, 1><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.slices, p.slice, a);         \
     else                                                                                       \
       knn_pass_kernel<D, 2><<<grid, 256, 0, s>>>(Qb, Rb, Q, R, p.slices, p.slice, a);         \

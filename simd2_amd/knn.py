@@ -118,3 +118,41 @@ def knn_bf16(query, ref, k, sqrt=False, stream=None):
         for t in (runner.ws, runner.stats, d, i):
             t.record_stream(s)
     return d, i, runner.stats
+
+
+def _main(argv=None):
+    """Time the precision form on Q = R points from 64 Gaussian clusters (the
+    PLD app's data): one untimed call per shape, then the mean over --reps
+    event-timed calls on the current stream."""
+    import argparse
+    from .datasets import gaussian_clusters
+    ap = argparse.ArgumentParser(prog="python -m simd2_amd.knn")
+    ap.add_argument("--sizes", type=int, nargs="+", default=[1024, 2048, 4096, 8192, 16384])
+    ap.add_argument("--k", type=int, nargs="+", default=[16])
+    ap.add_argument("--dim", type=int, default=128)
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args(argv)
+    torch = _torch()
+    for n in args.sizes:
+        ref, centers = gaussian_clusters(torch, n, args.dim, 64, seed=100)
+        qry, _ = gaussian_clusters(torch, n, args.dim, 64, seed=101, centers=centers)
+        runner = KnnBF16(n, n, args.dim)
+        for k in args.k:
+            runner.stats.zero_()
+            runner(qry, ref, k)
+            torch.cuda.synchronize()
+            stats = runner.stats.cpu().tolist()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.reps):
+                runner(qry, ref, k)
+            e1.record()
+            e1.synchronize()
+            print(f"Q=R={n} d={args.dim} k={k}: {e0.elapsed_time(e1) / args.reps:.4f} ms per call;"
+                  f" one call: {stats[0]} queries used FP64 ({stats[1]} candidates),"
+                  f" {stats[2]} took the exact scan", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

@@ -163,6 +163,8 @@ def fp64_topk_full(Qp, Rp, k):
                                           (130, 70, 96, 16, "gauss"),
                                           (200, 2100, 64, 16, "clusters"),
                                           (100, 40000, 32, 8, "clusters"),
+                                          (256, 10000, 128, 4, "clusters"),
+                                          (200, 12000, 200, 8, "gauss"),
                                           (1000, 3000, 128, 31, "offset"),
                                           (600, 5000, 128, 64, "clusters"),
                                           (300, 3000, 64, 100, "gauss"),
@@ -172,7 +174,8 @@ def fp64_topk_full(Qp, Rp, k):
                                           (256, 1000, 1000, 20, "offset")])
 def test_knn_bf16_vs_fp64(gpu, Q, R, d, k, data):
     """Top-k agreement with FP64 and the distance error, over the register
-    (d <= 160) and the streamed (d > 160) passes; "offset" data sit 1000 away
+    (d <= 160) and the streamed (d > 160) passes, each with 16-reference tile
+    minima (R < 256 (k + 16)) and 64-reference ones; "offset" data sit 1000 away
     from the origin (the centring keeps the expansion's error at the data's
     own scale); "uniform" is the reference PLD test's data (coordinates
     uniform in [0, 10), apps/pld/cuASR/test.cpp) at d = 4096."""
