@@ -489,7 +489,7 @@ def batched_graphs(torch, simd2_amd, count=4096, seed=0, ceiling=None):
         return s.elapsed_time(e)
 
     timed_solve()  # first launch of the graph
-    default_ms = min(timed_solve() for _ in range(2))
+    default_ms = float(np.median([timed_solve() for _ in range(3)]))  # as the tuned figure
     upload()
     t0 = time.perf_counter()
     b.autotune()

@@ -78,9 +78,11 @@ int simd2_batch_download(simd2_batch *batch, int g, void *dst, int64_t ld,
 int simd2_batch_buffer(simd2_batch *batch, int g, void **ptr, int64_t *ld);
 
 /* Times the tile shapes of every (op, size bin) on the uploaded graphs and
- * whole first rounds of the candidate schedules, keeps the fastest and
- * re-captures the solve graph.  Reads the uploaded matrices and leaves them
- * unchanged; synchronises `stream`. */
+ * whole first rounds of the candidate schedules, then whole solves of the
+ * schedule in use and the best candidates on a temporary device copy of
+ * the uploaded matrices (allocated for the call; skipped when that memory
+ * is not available), keeps the fastest and re-captures the solve graph.
+ * Leaves the uploaded matrices unchanged; synchronises `stream`. */
 int simd2_batch_autotune(simd2_batch *batch, hipStream_t stream);
 
 /* Solves every graph in place: one hipGraphLaunch on `stream`. */

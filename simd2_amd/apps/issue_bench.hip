@@ -16,6 +16,10 @@
 //   max_min_i32  2 x v_min_i32 + v_max3_i32 (max_min / min_max on order-preserving
 //              integer keys of the floats)
 //   and_or     v_and_or_b32                 (packed or_and: 32 terms per lane)
+//   mfma_i8    v_mfma_i32_16x16x64_i8       (the int8 minus_square form's
+//              products; rate x 32768 ops x CUs x clock = ops/s)
+//   lds_b128   ds_read_b128, 16 bytes a lane from 1 KB blocks of a 32 KB LDS
+//              array (rate x 1024 = LDS bytes read per CU per cycle)
 //
 // One launch per (mix, W) of 4 W waves per CU (one or two workgroups per CU).
 // Every wave reads the shader clock (s_memtime) around its loop and its
@@ -40,10 +44,12 @@ constexpr int ITERS = 2048;
 constexpr int CHAINS = 8;
 
 enum Mix { ADD, MIN3, MIN_PLUS, MAX_MUL, MAX_MIN, SUB_FMAC, MIN_I32, MIN3_I32, MAX_MIN_I32, AND_OR,
-           NMIX };
+           MFMA_I8, LDS_B128, NMIX };
 const char *kMixName[NMIX] = {"add",      "min3",    "min_plus", "max_mul",     "max_min",
-                              "sub_fmac", "min_i32", "min3_i32", "max_min_i32", "and_or"};
-const int kInstrPerChainIter[NMIX] = {1, 1, 3, 3, 3, 2, 1, 1, 3, 1};
+                              "sub_fmac", "min_i32", "min3_i32", "max_min_i32", "and_or",
+                              "mfma_i8",  "lds_b128"};
+const int kInstrPerChainIter[NMIX] = {1, 1, 3, 3, 3, 2, 1, 1, 3, 1, 1, 1};
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 // One loop iteration = one asm block over the 8 chains (the compiler puts an
 // s_nop after every asm statement, so one block per iteration keeps that to
@@ -88,10 +94,29 @@ __global__ void issue_kernel(float seed, float *sink, unsigned long long *t, int
     m[c] = seed - c;
   }
   const float a = seed * 0.5f, b = seed * 0.25f;
+  i32x4 acc[CHAINS];  // MFMA_I8 accumulators / LDS_B128 read sums
+#pragma unroll
+  for (int c = 0; c < CHAINS; ++c) acc[c] = i32x4{c, 0, 0, 0};
+  const int tid = (int)__builtin_amdgcn_workitem_id_x();
+  __shared__ __attribute__((aligned(16))) int lds[MIX == LDS_B128 ? 8192 : 4];
+  if constexpr (MIX == LDS_B128) {
+    for (int i = tid; i < 8192; i += (int)__builtin_amdgcn_workgroup_size_x()) lds[i] = i * 7;
+    __syncthreads();
+  }
   __builtin_amdgcn_s_barrier();
   const unsigned long long t0c = __builtin_amdgcn_s_memtime();
   for (int it = 0; it < ITERS; ++it) {
-    if constexpr (MIX == ADD)
+    if constexpr (MIX == MFMA_I8) {
+      const i32x4 av = {(int)seed, it, 3, 5}, bv = {7, (int)seed, it, 1};
+#pragma unroll
+      for (int c = 0; c < CHAINS; ++c) acc[c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[c], 0, 0, 0);
+    } else if constexpr (MIX == LDS_B128) {
+      // block (wave + c + it) % 32 of 1 KB, lane l at 16 l: in bounds, no conflicts
+#pragma unroll
+      for (int c = 0; c < CHAINS; ++c)
+        acc[c] ^= *reinterpret_cast<const i32x4 *>(
+            &lds[((tid / 64 + c + it) % 32) * 256 + (tid % 64) * 4]);
+    } else if constexpr (MIX == ADD)
       asm volatile(S2_CH(S2_ADD) : S2_OUT_M : S2_IN_Y);
     else if constexpr (MIX == MIN3)
       asm volatile(S2_CH(S2_MIN3) : S2_OUT_M : S2_IN_Y);
@@ -115,7 +140,7 @@ __global__ void issue_kernel(float seed, float *sink, unsigned long long *t, int
   const unsigned long long t1c = __builtin_amdgcn_s_memtime();
   float s = t0 + t1;
 #pragma unroll
-  for (int c = 0; c < CHAINS; ++c) s += m[c];
+  for (int c = 0; c < CHAINS; ++c) s += m[c] + (float)(acc[c][0] ^ acc[c][3]);
   if (s == 1234.5f) sink[0] = s;  // keeps the chains live; never true for the seeds used
   const int lane = (int)__builtin_amdgcn_workitem_id_x() % 64;
   const int wave = (int)__builtin_amdgcn_workitem_id_x() / 64;
@@ -215,7 +240,8 @@ int main(int argc, char **argv) {
         !run<MIN_I32>(cus, wps, p.clockRate, json) ||
         !run<MIN3_I32>(cus, wps, p.clockRate, json) ||
         !run<MAX_MIN_I32>(cus, wps, p.clockRate, json) ||
-        !run<AND_OR>(cus, wps, p.clockRate, json))
+        !run<AND_OR>(cus, wps, p.clockRate, json) || !run<MFMA_I8>(cus, wps, p.clockRate, json) ||
+        !run<LDS_B128>(cus, wps, p.clockRate, json))
       return 1;
   }
   return 0;

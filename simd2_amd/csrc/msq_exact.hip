@@ -17,11 +17,12 @@
 // The caller supplies the workspace (minus_square_exact_workspace bytes);
 // nothing here allocates.  Steps (one stream, no host synchronisation):
 //   1. quantize: X rows and Y columns to int8 in fragment order (below),
-//      zero-padded to 128 rows / columns and to a multiple of 64 in k, with
+//      zero-padded to 256 rows / 128 columns and to a multiple of 64 in k, with
 //      each row's / column's exact sum of squares (bit 31 set: some entry was
 //      not an integer in [-128, 127]);
-//   2. MFMA tiles 128x128 (4 waves x 64x64 quadrants of 4x4 16x16 blocks),
-//      64-deep int8 slabs double-buffered in LDS, epilogue S -> float [+ C];
+//   2. MFMA tiles 256x128 (8 waves x 64x64 quadrants of 4x4 16x16 blocks),
+//      64-deep int8 slabs copied straight into LDS, three in flight,
+//      epilogue S -> float [+ C];
 //   3. the VALU kernel on the 64x64 grid computes the tiles whose gate is 1
 //      (srgemm_valu.hip; every other workgroup returns at once).
 #include <stdint.h>
@@ -45,7 +46,10 @@ namespace {
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int QT = 128;        // MFMA output tile
+constexpr int QT = 128;        // rows (columns) of a fragment panel
+constexpr int TR = 256;        // MFMA tile rows: two X panels
+constexpr int TC = 128;        // MFMA tile columns: one Y panel
+constexpr int NST = 3;         // LDS stages: slabs in flight
 constexpr int QK = 64;         // k per slab = one MFMA k-step
 constexpr int QSLAB = QT * QK; // bytes of one 128-row panel slab (8 fragment blocks)
 constexpr int QR = 32;         // rows (or columns) per quantize workgroup
@@ -214,61 +218,104 @@ __device__ __forceinline__ void map_tile(int b, int tiles_r, int tiles_c, int &t
 // element (4 * (l / 16) + v, l % 16).  A and B take their k bytes in the same
 // order, and integer sums are exact, so the order of k within a group does
 // not change the result.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void msq_mfma_kernel(
+// One direct global -> LDS copy of 16 bytes per lane: lane l's bytes land at
+// lds + 16 l, lds being the wave's base (the same in every lane).  The copy
+// counts as an outstanding vector-memory load until the bytes are in LDS.
+__device__ __forceinline__ void lds_copy16(const int8_t *g, int8_t *lds) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
+                                   (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
+}
+
+// Wait until at most N of this thread's vector-memory loads are outstanding
+// (they complete in issue order) and its LDS reads have returned, then the
+// workgroup barrier.  The memory clobber keeps the compiler from moving
+// memory accesses across it; no fence is implied, so copies issued earlier
+// may still be in flight, as intended.
+template <int N>
+__device__ __forceinline__ void wait_copies_barrier() {
+  asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
+// The wave's four X and four Y fragments of a slab (16 bytes per lane, blocks
+// 1 KB apart), waited for in the same statement.  Written out because the
+// compiler, seeing LDS reads of memory that direct copies write, would wait
+// for every copy in flight, the next slabs' included, before each read.
+__device__ __forceinline__ void read_fragments(const int8_t *xs, const int8_t *ys, i32x4 (&a)[4],
+                                               i32x4 (&b)[4]) {
+  const uint32_t xo = (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) int8_t *)xs;
+  const uint32_t yo = (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) int8_t *)ys;
+  asm volatile(
+      "ds_read_b128 %0, %8\n\t"
+      "ds_read_b128 %1, %8 offset:1024\n\t"
+      "ds_read_b128 %2, %8 offset:2048\n\t"
+      "ds_read_b128 %3, %8 offset:3072\n\t"
+      "ds_read_b128 %4, %9\n\t"
+      "ds_read_b128 %5, %9 offset:1024\n\t"
+      "ds_read_b128 %6, %9 offset:2048\n\t"
+      "ds_read_b128 %7, %9 offset:3072\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(a[0]), "=&v"(a[1]), "=&v"(a[2]), "=&v"(a[3]), "=&v"(b[0]), "=&v"(b[1]),
+        "=&v"(b[2]), "=&v"(b[3])
+      : "v"(xo), "v"(yo)
+      : "memory");
+}
+
+// Tiles of 256 rows x 128 columns, 8 waves of 64x64 quadrants.  The 64-k
+// slabs (two X panels' 8 KB runs, one Y panel's) are copied straight into
+// LDS, three slabs in flight: slab ks + 2 is issued right after the barrier
+// that publishes slab ks, into the stage slab ks - 1 left, so each slab has
+// two slabs' products to arrive in.  72 KB of LDS: two workgroups per CU.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void msq_mfma_kernel(
     int R, int Cn, int Kp, const int8_t *__restrict__ Xq, const int8_t *__restrict__ Yq,
     const int *__restrict__ nx, const int *__restrict__ ny, float *O, int64_t ldo,
     const float *C, int64_t ldc, int *__restrict__ gate, int gate_ld, int tiles_r,
     int tiles_c, bool vec_out) {
-  __shared__ __attribute__((aligned(16))) int8_t Xs[2][QSLAB];
-  __shared__ __attribute__((aligned(16))) int8_t Ys[2][QSLAB];
-  // Quadrant flags (rows 0-63, rows 64-127, columns 0-63, columns 64-127) in
-  // the second X buffer, which nothing writes before the first slab's barrier.
-  int *qbad = reinterpret_cast<int *>(&Xs[1][0]);
+  // stage st: the X slab (two panels) at st * 3 QSLAB, the Y slab after it
+  __shared__ __attribute__((aligned(16))) int8_t smem[NST * 3 * QSLAB];
+  __shared__ int qbad[6];  // row quadrants 0-3, column quadrants 0-1
+  __shared__ int badw[8];  // waves with a sum past 2^24
   const int tid = tid_x();
   const int lane = tid % 64;
   const int wave = tid / 64;
   const int wr = wave / 2, wc = wave % 2;
   int tr, tc;
   map_tile(bid_x(), tiles_r, tiles_c, tr, tc);
-  const int row0 = tr * QT, col0 = tc * QT;
+  const int row0 = tr * TR, col0 = tc * TC;
 
   // Quadrants with a row or column that failed the integer check go to the
-  // VALU kernel; when all four do, skip the products.
-  if (tid < 4) qbad[tid] = 0;
+  // VALU kernel; when all eight do, skip the products.
+  if (tid < 6) qbad[tid] = 0;
   __syncthreads();
-  {
-    const int t = tid % QT;
-    if (tid < QT) {
-      const int r = row0 + t;
-      if (r < R && nx[r] < 0) qbad[t / 64] = 1;
-    } else {
-      const int c = col0 + t;
-      if (c < Cn && ny[c] < 0) qbad[2 + t / 64] = 1;
-    }
+  if (tid < TR) {
+    const int r = row0 + tid;
+    if (r < R && nx[r] < 0) qbad[tid / 64] = 1;
+  } else if (tid < TR + TC) {
+    const int c = col0 + tid - TR;
+    if (c < Cn && ny[c] < 0) qbad[4 + (tid - TR) / 64] = 1;
   }
   __syncthreads();
-  int *g = gate + (int64_t)(tr * 2 + wr) * gate_ld + tc * 2 + wc;
-  const bool mybad = qbad[wr] | qbad[2 + wc];
-  if ((qbad[0] | qbad[2]) & (qbad[0] | qbad[3]) & (qbad[1] | qbad[2]) & (qbad[1] | qbad[3])) {
+  int *g = gate + (int64_t)(tr * 4 + wr) * gate_ld + tc * 2 + wc;
+  const bool mybad = qbad[wr] | qbad[4 + wc];
+  bool allbad = true;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) allbad = allbad && (qbad[q] | qbad[4]) && (qbad[q] | qbad[5]);
+  if (allbad) {
     if (lane == 0) *g = 1;
     return;
   }
 
-  // slab ks of the tile's row panel: Xq + (tr * Kp / 64 + ks) * 8 KB
-  const int8_t *xp = Xq + (int64_t)tr * (Kp / QK) * QSLAB + tid * 16;
-  const int8_t *yp = Yq + (int64_t)tc * (Kp / QK) * QSLAB + tid * 16;
-  uint4 x0, x1, y0, y1;
-  auto load = [&]() {
-    x0 = *reinterpret_cast<const uint4 *>(xp);
-    x1 = *reinterpret_cast<const uint4 *>(xp + QSLAB / 2);
-    y0 = *reinterpret_cast<const uint4 *>(yp);
-    y1 = *reinterpret_cast<const uint4 *>(yp + QSLAB / 2);
-  };
-  auto stage = [&](int b) {
-    *reinterpret_cast<uint4 *>(&Xs[b][tid * 16]) = x0;
-    *reinterpret_cast<uint4 *>(&Xs[b][QSLAB / 2 + tid * 16]) = x1;
-    *reinterpret_cast<uint4 *>(&Ys[b][tid * 16]) = y0;
-    *reinterpret_cast<uint4 *>(&Ys[b][QSLAB / 2 + tid * 16]) = y1;
+  // Wave w copies fragment block w of each slab: of X panels 2 tr and
+  // 2 tr + 1 (Rq is a multiple of 256, so both exist) and of Y panel tc.
+  const int nks = Kp / QK;
+  const int8_t *xa = Xq + (int64_t)(2 * tr) * nks * QSLAB + wave * 1024 + lane * 16;
+  const int8_t *xb = xa + (int64_t)nks * QSLAB;
+  const int8_t *ya = Yq + (int64_t)tc * nks * QSLAB + wave * 1024 + lane * 16;
+  auto issue = [&](int ks, int st) {
+    const int64_t o = (int64_t)ks * QSLAB;
+    int8_t *x = &smem[st * 3 * QSLAB];
+    lds_copy16(xa + o, x + wave * 1024);
+    lds_copy16(xb + o, x + QSLAB + wave * 1024);
+    lds_copy16(ya + o, x + 2 * QSLAB + wave * 1024);
   };
 
   i32x4 acc[4][4];
@@ -277,34 +324,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void m
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
 
-  const int nks = Kp / QK;
-  load();
-  stage(0);
-  __syncthreads();
-  const int fr = lane % 16;
+  issue(0, 0);
+  if (nks > 1) issue(1, 1);
+  int st = 0;
   for (int ks = 0; ks < nks; ++ks) {
-    const int buf = ks & 1;
-    const bool more = ks + 1 < nks;
-    if (more) {
-      xp += QSLAB;
-      yp += QSLAB;
-      load();
-    }
+    // slab ks is in LDS once at most the next slab's 3 copies are outstanding
+    if (ks + 1 < nks)
+      wait_copies_barrier<3>();
+    else
+      wait_copies_barrier<0>();
+    // every wave read slab ks - 1 before this barrier: its stage takes ks + 2
+    if (ks + 2 < nks) issue(ks + 2, st == 0 ? 2 : st - 1);
     i32x4 a[4], b[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      a[i] = *reinterpret_cast<const i32x4 *>(&Xs[buf][(wr * 4 + i) * 1024 + lane * 16]);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      b[j] = *reinterpret_cast<const i32x4 *>(&Ys[buf][(wc * 4 + j) * 1024 + lane * 16]);
+    const int8_t *x = &smem[st * 3 * QSLAB];
+    read_fragments(x + wr * 4 * 1024 + lane * 16, x + 2 * QSLAB + wc * 4 * 1024 + lane * 16, a, b);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
-    if (more) stage(buf ^ 1);
-    __syncthreads();
+    st = st == NST - 1 ? 0 : st + 1;
   }
+  // the epilogue reuses the stages: every wave's reads of the last slab come first
+  wait_copies_barrier<0>();
+  const int fr = lane % 16;
 
   // Epilogue of this wave's 64x64 quadrant: first whether every sum is
   // exact in FP32, then the stores.
@@ -315,6 +358,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void m
     const int c = qc0 + j * 16 + fr;
     nyv[j] = c < Cn ? ny[c] : 0;
   }
+  // Lane l holds the norm of the quadrant's row l (one load per lane); the
+  // accumulator rows take theirs by a lane permute.
+  const uint32_t nx_lane = qr0 + lane < R ? (uint32_t)nx[qr0 + lane] : 0u;
+  auto row_norm = [&](int i, int v) {
+    return (uint32_t)__shfl((int)nx_lane, i * 16 + (lane / 16) * 4 + v);
+  };
   // For valid rows and columns the sum s = sum_k (x - y)^2 is below
   // 65536 * 255^2 < 2^32, so it is exact in 32-bit unsigned arithmetic.
   bool bad = mybad;
@@ -323,34 +372,95 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void m
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = qr0 + i * 16 + (lane / 16) * 4 + v;
-      const uint32_t nxv = r < R ? (uint32_t)nx[r] : 0u;
+      const uint32_t nxv = row_norm(i, v);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int c = qc0 + j * 16 + fr;
         const uint32_t s = nxv + (uint32_t)nyv[j] - 2u * (uint32_t)acc[i][j][v];
-        if (r < R && c < Cn && s > (uint32_t)EXACT_LIMIT) bad = true;
+        bad = bad | ((r < R) & (c < Cn) & (s > (uint32_t)EXACT_LIMIT));
       }
     }
-  if (__ballot(bad) != 0) {
+  const bool wave_bad = __ballot(bad) != 0;
+
+  // Interior tiles with C (rows 16-byte aligned): C is copied straight into
+  // LDS, half the tile at a time (rows wr * 64 + h * 32 .. + 31 of every row
+  // quadrant: 128 rows x 512 bytes, two rows per 1 KB copy, 1056 bytes apart
+  // so that the 4 row groups of an accumulator-layout access fall in distinct
+  // banks); each wave adds its exact sums in place and stores its 32 x 64
+  // block from there, 16 bytes per lane along rows.  The whole half's C is in
+  // flight at once, where loading it per 16-row block waited for memory 4
+  // times in turn.  Taken only when no wave of the tile failed the check.
+  if (C && vec_out && row0 + TR <= R && col0 + TC <= Cn) {
+    if (lane == 0) badw[wave] = wave_bad ? 1 : 0;
+    __syncthreads();
+    int anybad = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) anybad |= badw[w];
+    if (!anybad) {
+      constexpr int CP = 1056;
+      static_assert(64 * CP <= NST * 3 * QSLAB, "half a C tile exceeds the stages");
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h) wait_copies_barrier<0>();  // every wave has stored the first half
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int m = wave * 8 + q;        // copy m: local rows 2m, 2m + 1
+          const int lr = 2 * m + lane / 32;  // local row l: tile row (l/32)*64 + h*32 + l%32
+          const int trow = (lr / 32) * 64 + h * 32 + lr % 32;
+          lds_copy16(reinterpret_cast<const int8_t *>(C + (int64_t)(row0 + trow) * ldc + col0 +
+                                                      (lane % 32) * 4),
+                     &smem[m * CP]);
+        }
+        wait_copies_barrier<0>();
+#pragma unroll
+        for (int ii = 0; ii < 2; ++ii) {
+          const int i = 2 * h + ii;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rr = (lane / 16) * 4 + v;
+            const int lr = wr * 32 + ii * 16 + rr;
+            const uint32_t nxv = row_norm(i, v);
+            float *pc = reinterpret_cast<float *>(&smem[(lr / 2) * CP + (lr % 2) * 512]) + wc * 64 + fr;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)  // the gate loop's exact sum, <= 2^24
+              pc[j * 16] += (float)(nxv + (uint32_t)nyv[j] - 2u * (uint32_t)acc[i][j][v]);
+          }
+        }
+        // only this wave wrote its block, and its LDS operations complete in order
+#pragma unroll
+        for (int hh = 0; hh < 8; ++hh) {
+          const int idx = lane + 64 * hh;
+          const int row = idx / 16, cc = (idx % 16) * 4;
+          const int lr = wr * 32 + row;
+          const f32x4 d = *reinterpret_cast<const f32x4 *>(
+              &smem[(lr / 2) * CP + (lr % 2) * 512 + (wc * 64 + cc) * 4]);
+          *reinterpret_cast<f32x4 *>(O + (int64_t)(qr0 + h * 32 + row) * ldo + qc0 + cc) = d;
+        }
+      }
+      if (lane == 0) *g = 0;
+      return;
+    }
+  }
+  if (wave_bad) {
     if (lane == 0) *g = 1;
     return;
   }
   // Stores through LDS, one 16-row block of the quadrant at a time: the wave
   // writes the block's exact sums (as floats) to its own LDS region in the
   // accumulator layout and reads them back row-major, so the C loads and the
-  // stores are 16 bytes per lane along rows.  The main loop's last barrier
-  // has retired every read of Xs / Ys; only this wave touches its region, and
-  // a wave's LDS operations complete in order, so no workgroup barrier is
-  // needed (a wave may have left above).
+  // stores are 16 bytes per lane along rows.  The barrier after the main loop
+  // has retired every read of the stages; only this wave touches its region
+  // (8 x 4352 bytes), and a wave's LDS operations complete in order, so no
+  // workgroup barrier is needed (a wave may have left above).
   constexpr int TLD = 68;  // padded row: the 4 row groups of a store fall in distinct banks
-  float *T = reinterpret_cast<float *>(wave < 2 ? &Xs[0][0] : &Ys[0][0]) + (wave & 1) * 16 * TLD;
+  static_assert(8 * 16 * TLD * 4 <= NST * 3 * QSLAB, "store regions exceed the stages");
+  float *T = reinterpret_cast<float *>(&smem[0]) + wave * 16 * TLD;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int rr = (lane / 16) * 4 + v;
-      const int r = qr0 + i * 16 + rr;
-      const uint32_t nxv = r < R ? (uint32_t)nx[r] : 0u;
+      const uint32_t nxv = row_norm(i, v);
 #pragma unroll
       for (int j = 0; j < 4; ++j)  // the gate loop's exact unsigned sum, <= 2^24 where stored
         T[rr * TLD + j * 16 + fr] = (float)(nxv + (uint32_t)nyv[j] - 2u * (uint32_t)acc[i][j][v]);
@@ -391,8 +501,8 @@ size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 WsLayout ws_layout(int R, int Cn, int K) {
   WsLayout w;
   w.Kp = (K + QK - 1) / QK * QK;
-  w.Rq = (R + QT - 1) / QT * QT;
-  w.Cq = (Cn + QT - 1) / QT * QT;
+  w.Rq = (R + TR - 1) / TR * TR;
+  w.Cq = (Cn + TC - 1) / TC * TC;
   w.gate_ld = w.Cq / 64;
   w.gate_n = (w.Rq / 64) * w.gate_ld;
   w.bx = align256((size_t)w.Rq * w.Kp);
@@ -445,10 +555,10 @@ hipError_t launch_minus_square_exact(int R, int Cn, int K, const float *X, int64
                                                              Yq, nx, ny, qbx, kchunk);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int tiles_r = w.Rq / QT, tiles_c = w.Cq / QT;
+  const int tiles_r = w.Rq / TR, tiles_c = w.Cq / TC;
   auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
   const bool vec_out = ldo % 4 == 0 && aligned(O) && (!C || (ldc % 4 == 0 && aligned(C)));
-  msq_mfma_kernel<<<tiles_r * tiles_c, 256, 0, s>>>(R, Cn, w.Kp, Xq, Yq, nx, ny, O, ldo, C, ldc,
+  msq_mfma_kernel<<<tiles_r * tiles_c, 512, 0, s>>>(R, Cn, w.Kp, Xq, Yq, nx, ny, O, ldo, C, ldc,
                                                     gate, w.gate_ld, tiles_r, tiles_c, vec_out);
   e = hipGetLastError();
   if (e != hipSuccess) return e;

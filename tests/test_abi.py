@@ -91,15 +91,15 @@ def test_generic_entry_point_errors(lib):
 
 
 def test_minus_square_exact_workspace_and_argument_errors(lib):
-    """The workspace size (int8 panels padded to 128 rows / columns and to a
-    multiple of 64 in k, norms, tile gates, each 256-byte aligned) and the
-    host-side refusals, none of which reaches the GPU."""
+    """The workspace size (int8 panels padded to 256 X rows / 128 Y columns
+    and to a multiple of 64 in k, norms, tile gates, each 256-byte aligned)
+    and the host-side refusals, none of which reaches the GPU."""
     f = lib.simd2_minus_square_workspace
     a256 = lambda b: (b + 255) // 256 * 256
 
     def want(M, N, K):  # column-major M x N: the row-major N x M problem
         R, Cn, Kp = N, M, (K + 63) // 64 * 64
-        Rq, Cq = (R + 127) // 128 * 128, (Cn + 127) // 128 * 128
+        Rq, Cq = (R + 255) // 256 * 256, (Cn + 127) // 128 * 128
         return (a256(Rq * Kp) + a256(Cq * Kp) + a256(4 * R) + a256(4 * Cn) +
                 a256(4 * (Rq // 64) * (Cq // 64)))
     for M, N, K in ((300, 200, 500), (1, 1, 1), (128, 129, 64), (4096, 4096, 4096),
