@@ -219,8 +219,11 @@ hipError_t launch_permute_copy(float *dst, int64_t ldd, const float *src, int64_
 
 // Learned schedule (learned.hip).
 size_t graph_features_workspace(int n);
+// With model (params, then the normalisation: simd2_converge_learned's
+// layout), pred receives its prediction from the same launch.
 hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int epi, float *f,
-                                 void *ws, hipStream_t s);
+                                 void *ws, hipStream_t s, const float *model = nullptr,
+                                 float *pred = nullptr);
 // after_features: launch_graph_features already ran on this workspace (its
 // packed adjacency and search from vertex 0 are reused).
 hipError_t launch_graph_levels(int n, const float *X, int64_t ld, int *level, void *ws,

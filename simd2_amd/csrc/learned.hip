@@ -35,9 +35,14 @@ __device__ __forceinline__ bool is_edge(float v, int i, int j) {
 // When adj is set, the row is also packed there (bit (j & 31) of word j / 32:
 // edge (i, j)), one ballot per 64 entries.
 __global__ __launch_bounds__(256) void row_stats_kernel(const float *X, int64_t ld, int n,
-                                                        double *stats, uint32_t *adj, int nw) {
+                                                        double *stats, uint32_t *adj, int nw,
+                                                        int *indeg, int *sym) {
   __shared__ double sh[RS][256];
   const int i = bid_x(), t = tid_x();
+  // clears the in-degree and symmetry counts the next two kernels add into
+  // (a launch each otherwise, and launches, not work, bound small graphs)
+  if (t == 0) indeg[i] = 0;
+  if (i == 0 && t < 2) sym[t] = 0;
   const int lane = t % 64;
   double deg = 0, ws = 0, wq = 0, wmin = 1e300, wmax = 0, ds = 0, dmax = 0, near = 0;
   for (int j0 = 0; j0 < n; j0 += 256) {
@@ -374,10 +379,36 @@ __global__ __launch_bounds__(BFS_T) void bfs_levels_kernel(const uint32_t *adj, 
 
 // The 24 features (one workgroup; fixed-order reductions in double).
 constexpr int NRED = 13;
+// The MLP's layers on one normalised sample xs (LDS): thread j < NH computes
+// hidden unit j; every thread of the workgroup calls it (it has barriers).
+__device__ __forceinline__ void mlp_layers(const float *P, const float *xs, float *h1, float *h2,
+                                           float *y, const float *norm, int j) {
+  if (j < NH) {
+    float a = P[OFF_B1 + j];
+    for (int f = 0; f < NF; ++f) a += P[OFF_W1 + j * NF + f] * xs[f];
+    h1[j] = a > 0.f ? a : 0.f;
+  }
+  __syncthreads();
+  if (j < NH) {
+    float a = P[OFF_B2 + j];
+    for (int k = 0; k < NH; ++k) a += P[OFF_W2 + j * NH + k] * h1[k];
+    h2[j] = a > 0.f ? a : 0.f;
+  }
+  __syncthreads();
+  if (j < NO) {
+    float o = P[OFF_B3 + j];
+    for (int k = 0; k < NH; ++k) o += P[OFF_W3 + j * NH + k] * h2[k];
+    y[j] = norm ? o * norm[2 * NF + j] + norm[2 * NF + NO + j] : o;
+  }
+}
+
 __global__ __launch_bounds__(256) void features_kernel(const double *rstats, const int *indeg,
                                                        const int *sym, const int *bfs_res, int n,
-                                                       int max_based, int epi, float *f) {
+                                                       int max_based, int epi, float *f,
+                                                       const float *P, float *y,
+                                                       const float *norm) {
   __shared__ double sh[NRED][256];
+  __shared__ float fx[NF], h1[NH], h2[NH];
   const int t = tid_x();
   double v[NRED] = {0, 0, 0, 0, 0, 0, 1e300, 0, 0, 0, 0, 0, 0};
   for (int i = t; i < n; i += 256) {
@@ -407,49 +438,60 @@ __global__ __launch_bounds__(256) void features_kernel(const double *rstats, con
       }
     __syncthreads();
   }
-  if (t != 0) return;
-  const double N = n, m = sh[0][0];
-  const double mean = m / N;
-  const double sd_out = sqrt(fmax(sh[1][0] / N - mean * mean, 0.0));
-  const double sd_in = sqrt(fmax(sh[12][0] / N - mean * mean, 0.0));
-  const double density = n > 1 ? m / (N * (N - 1)) : 0.0;
-  const double wmax = sh[7][0];
-  const double wmean = m > 0 ? sh[4][0] / m : 0.0;
-  const double wsd = m > 0 ? sqrt(fmax(sh[5][0] / m - wmean * wmean, 0.0)) : 0.0;
-  double dmean = 0, dmx = 0, reach = 0, width = 0;
-  for (int k = 0; k < NSRC; ++k) {
-    const double d = bfs_res[3 * k], r = bfs_res[3 * k + 1];
-    dmean += d / NSRC;
-    dmx = fmax(dmx, d);
-    reach += r / N / NSRC;
-    width += r / (d + 1) / N / NSRC;
+  if (t == 0) {
+    const double N = n, m = sh[0][0];
+    const double mean = m / N;
+    const double sd_out = sqrt(fmax(sh[1][0] / N - mean * mean, 0.0));
+    const double sd_in = sqrt(fmax(sh[12][0] / N - mean * mean, 0.0));
+    const double density = n > 1 ? m / (N * (N - 1)) : 0.0;
+    const double wmax = sh[7][0];
+    const double wmean = m > 0 ? sh[4][0] / m : 0.0;
+    const double wsd = m > 0 ? sqrt(fmax(sh[5][0] / m - wmean * wmean, 0.0)) : 0.0;
+    double dmean = 0, dmx = 0, reach = 0, width = 0;
+    for (int k = 0; k < NSRC; ++k) {
+      const double d = bfs_res[3 * k], r = bfs_res[3 * k + 1];
+      dmean += d / NSRC;
+      dmx = fmax(dmx, d);
+      reach += r / N / NSRC;
+      width += r / (d + 1) / N / NSRC;
   }
-  double out[NF];
-  out[0] = log2(N) / 16;
-  out[1] = density;
-  out[2] = (log10(density + 1e-9) + 9) / 9;
-  out[3] = log2(1 + mean) / 16;
-  out[4] = sd_out / (1 + mean);
-  out[5] = log2(1 + sh[2][0]) / 16;
-  out[6] = sh[3][0] / N;
-  out[7] = sh[11][0] / N;
-  out[8] = sd_in / (1 + mean);
-  out[9] = wmax > 0 ? wmean / wmax : 0;
-  out[10] = m > 0 ? fmin(wsd / (wmean + 1e-12), 10.0) : 0;
-  out[11] = (m > 0 && wmax > 0) ? sh[6][0] / wmax : 0;
-  out[12] = log10(1 + wmax) / 10;
-  out[13] = sym[0] > 0 ? (double)sym[1] / sym[0] : 0;
-  out[14] = m > 0 ? sh[8][0] / m / N : 0;
-  out[15] = m > 0 ? sh[9][0] / N : 0;
-  out[16] = m > 0 ? sh[10][0] / m : 0;
-  out[17] = log2(1 + dmean) / 16;
-  out[18] = log2(1 + dmx) / 16;
-  out[19] = reach;
-  out[20] = width;
-  out[21] = max_based ? 1 : 0;
-  out[22] = epi ? 1 : 0;
-  out[23] = (ceil(log2(fmax(dmx, 1.0))) + 1) / 16;
-  for (int k = 0; k < NF; ++k) f[k] = (float)out[k];
+    double out[NF];
+    out[0] = log2(N) / 16;
+    out[1] = density;
+    out[2] = (log10(density + 1e-9) + 9) / 9;
+    out[3] = log2(1 + mean) / 16;
+    out[4] = sd_out / (1 + mean);
+    out[5] = log2(1 + sh[2][0]) / 16;
+    out[6] = sh[3][0] / N;
+    out[7] = sh[11][0] / N;
+    out[8] = sd_in / (1 + mean);
+    out[9] = wmax > 0 ? wmean / wmax : 0;
+    out[10] = m > 0 ? fmin(wsd / (wmean + 1e-12), 10.0) : 0;
+    out[11] = (m > 0 && wmax > 0) ? sh[6][0] / wmax : 0;
+    out[12] = log10(1 + wmax) / 10;
+    out[13] = sym[0] > 0 ? (double)sym[1] / sym[0] : 0;
+    out[14] = m > 0 ? sh[8][0] / m / N : 0;
+    out[15] = m > 0 ? sh[9][0] / N : 0;
+    out[16] = m > 0 ? sh[10][0] / m : 0;
+    out[17] = log2(1 + dmean) / 16;
+    out[18] = log2(1 + dmx) / 16;
+    out[19] = reach;
+    out[20] = width;
+    out[21] = max_based ? 1 : 0;
+    out[22] = epi ? 1 : 0;
+    out[23] = (ceil(log2(fmax(dmx, 1.0))) + 1) / 16;
+    for (int k = 0; k < NF; ++k) {
+      f[k] = (float)out[k];
+      fx[k] = (float)out[k];
+    }
+  }
+  // With a model: its prediction from the same values, as mlp_forward_kernel
+  // would compute it from f (one launch fewer).
+  if (!P) return;
+  __syncthreads();
+  if (t < NF && norm) fx[t] = (fx[t] - norm[t]) * norm[NF + t];
+  __syncthreads();
+  mlp_layers(P, fx, h1, h2, y, norm, t);
 }
 
 // ---- MLP -------------------------------------------------------------------
@@ -466,19 +508,7 @@ __global__ __launch_bounds__(64) void mlp_forward_kernel(const float *P, const f
     xs[j] = norm ? (v - norm[j]) * norm[NF + j] : v;
   }
   __syncthreads();
-  float a = P[OFF_B1 + j];
-  for (int f = 0; f < NF; ++f) a += P[OFF_W1 + j * NF + f] * xs[f];
-  h1[j] = a > 0.f ? a : 0.f;
-  __syncthreads();
-  a = P[OFF_B2 + j];
-  for (int k = 0; k < NH; ++k) a += P[OFF_W2 + j * NH + k] * h1[k];
-  h2[j] = a > 0.f ? a : 0.f;
-  __syncthreads();
-  if (j < NO) {
-    float o = P[OFF_B3 + j];
-    for (int k = 0; k < NH; ++k) o += P[OFF_W3 + j * NH + k] * h2[k];
-    y[(int64_t)b * NO + j] = norm ? o * norm[2 * NF + j] + norm[2 * NF + NO + j] : o;
-  }
+  mlp_layers(P, xs, h1, h2, y + (int64_t)b * NO, norm, j);
 }
 
 // Gradient of L = (1/B) sum_b sum_o w_o (y_bo - t_bo)^2 over GRAD_CHUNK
@@ -609,7 +639,7 @@ hipError_t pack_adjacency(const float *X, int64_t ld, int n, uint32_t *adj, hipS
 size_t graph_features_workspace(int n) { return n > 0 ? feat_layout(n).total : 0; }
 
 hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int epi, float *f,
-                                 void *ws, hipStream_t s) {
+                                 void *ws, hipStream_t s, const float *model, float *pred) {
   const FeatLayout L = feat_layout(n);
   char *w = static_cast<char *>(ws);
   double *rstats = reinterpret_cast<double *>(w + L.rstats);
@@ -619,24 +649,22 @@ hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int 
   int *lists = reinterpret_cast<int *>(w + L.lists);
   int *res = reinterpret_cast<int *>(w + L.res);
   const int nw = (n + 31) / 32;
-  hipError_t e = launch_fill_i32(indeg, 0, n, s);
-  if (e == hipSuccess) e = launch_fill_i32(sym, 0, 2, s);
-  if (e != hipSuccess) return e;
-  row_stats_kernel<<<dim3(n), dim3(256), 0, s>>>(X, ld, n, rstats, adj, nw);
+  row_stats_kernel<<<dim3(n), dim3(256), 0, s>>>(X, ld, n, rstats, adj, nw, indeg, sym);
   // about 2048 workgroups whatever n: each thread walks `chunk` rows of one column
   const int chunk = std::max(8, (int)(((int64_t)n * n) / (256 * 2048)));
   in_degree_kernel<<<dim3((n + 255) / 256, (n + chunk - 1) / chunk), dim3(256), 0, s>>>(
       X, ld, n, chunk, indeg);
   const int step = std::max(1, n / 256);
   symmetry_kernel<<<dim3((n + step - 1) / step), dim3(256), 0, s>>>(adj, n, nw, step, sym);
-  e = hipGetLastError();
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (nw <= SB_NW)
     bfs_sources_small_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, res);
   else
     bfs_sources_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, lists, res);
   const int max_based = (op == 2 || op == 3 || op == 4) ? 1 : 0;
-  features_kernel<<<dim3(1), dim3(256), 0, s>>>(rstats, indeg, sym, res, n, max_based, epi, f);
+  features_kernel<<<dim3(1), dim3(256), 0, s>>>(rstats, indeg, sym, res, n, max_based, epi, f,
+                                                 model, pred, model ? model + NPARAM : nullptr);
   return hipGetLastError();
 }
 

@@ -154,9 +154,8 @@ int simd2_converge_learned(int op, int n, float *X, int ldx, int do_epilogue, in
   char *sched_ws = reinterpret_cast<char *>(perm) + align256((size_t)n * sizeof(int));
   const size_t sched_bytes = workspace_bytes - (size_t)(sched_ws - ws);
 
-  hipError_t e = launch_graph_features(op, n, X, ldx, do_epilogue, feat, ws, stream);
-  if (e == hipSuccess)
-    e = launch_mlp_forward(model, feat, 1, pred, model + SIMD2_MLP_PARAMS, stream);
+  // features and the network's prediction in one launch sequence
+  hipError_t e = launch_graph_features(op, n, X, ldx, do_epilogue, feat, ws, stream, model, pred);
   // features and predictions in one copy: pred follows feat in the workspace
   constexpr size_t pred_at = (SIMD2_FEATURES * 4 + 255) / 256 * 256;
   float hostbuf[pred_at / 4 + SIMD2_MLP_OUT];
