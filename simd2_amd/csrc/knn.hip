@@ -304,7 +304,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const __bf16 *__restrict__ Qb, const __bf16 *__restrict__ Rb, int Q, int R, int slices,
     int slice, PassArgs a) {
   constexpr int KS = DPAD / 32;             // MFMA k-steps
-  constexpr int RLD = 2 * DPAD + 8;         // LDS row: hi | lo, +16 bytes of padding
+  // LDS row: hi | lo, +32 bytes of padding: a row pitch of DPAD + 8 words
+  // (8 mod 16) keeps the 16 rows of a fragment read on distinct banks; the
+  // 16-byte padding (4 mod 16) cost twice the LDS cycles in conflicts
+  constexpr int RLD = 2 * DPAD + 16;
   constexpr int CHUNKS = CR * 2 * DPAD / 8; // 16-byte chunks per reference tile
   constexpr int PER = CHUNKS / 256;         // chunks per thread
   __shared__ __attribute__((aligned(16))) __bf16 Rt[CR][RLD];
