@@ -59,6 +59,68 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
                                             flag, sm);
 }
 
+// Small problems (fewer 64x64 tiles than two per CU; n = 1024 has 256): the
+// K loop of each tile is split over SK_GROUPS groups of 256 threads in one
+// workgroup, each over a contiguous range of slabs with its own LDS buffers,
+// and the partial tiles are combined in group order, ((p0 (+) p1) (+) p2) (+)
+// p3, before the usual epilogue: four times the resident waves (one per SIMD
+// otherwise).  Only for the semirings whose (+) is min or max, which is
+// exact: regrouping the terms changes no bit.
+constexpr int SK_GROUPS = 4;
+
+template <class Op>
+__global__ __launch_bounds__(256 * SK_GROUPS) void srgemm_valu_splitk_kernel(
+    int R, int Cn, int K, const float *__restrict__ X, int64_t ldx,
+    const float *__restrict__ Y, int64_t ldy, float *O, int64_t ldo, Epilogue e,
+    int tiles_r, int tiles_c) {
+  using S = CfgS;
+  __shared__ ValuSmem<S::BM, S::BN, S::BK> sm[SK_GROUPS];
+  int tr, tc;
+  map_tile(bid_x(), tiles_r, tiles_c, tr, tc);
+  int *flag = e.tile_flags ? &e.tile_flags[tr * tiles_c + tc] : nullptr;
+  const int row0 = tr * S::BM, col0 = tc * S::BN;
+  const int g = tid_x() / 256, t = tid_x() % 256;
+  if (tid_x() == 0) sm[0].changed = 0;  // published by the main loop's barriers
+  const int per = K / S::BK / SK_GROUPS;
+  float acc[S::TM][S::TN];
+  valu_mainloop<Op, S::BM, S::BN, S::TM, S::TN, S::BK, false, false>(
+      R, Cn, K, X, ldx, Y, ldy, row0, col0, nullptr, 0, sm[g], acc, t, g * per, (g + 1) * per);
+  // The main loop ends with a barrier after every read of the slabs: groups
+  // 1.. park their partial tiles in their own LDS (16 KB of its 16.5).
+  static_assert(S::TM * S::TN * 256 * sizeof(float) <= sizeof(ValuSmem<S::BM, S::BN, S::BK>) -
+                                                           sizeof(int), "partial tile");
+  if (g > 0) {
+    float *part = reinterpret_cast<float *>(&sm[g]);
+#pragma unroll
+    for (int i = 0; i < S::TM; ++i)
+#pragma unroll
+      for (int j = 0; j < S::TN; ++j) part[(i * S::TN + j) * 256 + t] = acc[i][j];
+  }
+  __syncthreads();
+  if (g == 0) {
+#pragma unroll
+    for (int q = 1; q < SK_GROUPS; ++q) {
+      const float *part = reinterpret_cast<const float *>(&sm[q]);
+#pragma unroll
+      for (int i = 0; i < S::TM; ++i)
+#pragma unroll
+        for (int j = 0; j < S::TN; ++j)
+          acc[i][j] = Op::add(acc[i][j], part[(i * S::TN + j) * 256 + t]);
+    }
+  }
+  valu_epilogue<Op, S::BM, S::BN, S::TM, S::TN, false, false>(R, Cn, O, ldo, e, row0, col0, flag,
+                                                              &sm[0].changed, acc, t, g == 0);
+}
+
+template <class Op>
+hipError_t launch_splitk(int R, int Cn, int K, const float *X, int64_t ldx, const float *Y,
+                         int64_t ldy, float *O, int64_t ldo, const Epilogue &e, hipStream_t s) {
+  const int tiles_r = R / CfgS::BM, tiles_c = Cn / CfgS::BN;
+  srgemm_valu_splitk_kernel<Op><<<dim3(tiles_r * tiles_c), dim3(256 * SK_GROUPS), 0, s>>>(
+      R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
+  return hipGetLastError();
+}
+
 template <class Op, class Cfg>
 hipError_t launch_cfg(int R, int Cn, int K, const float *X, int64_t ldx,
                       const float *Y, int64_t ldy, float *O, int64_t ldo,
@@ -161,11 +223,39 @@ int valu_pick_cfg(int op, int R, int Cn, int K) {
   return CFG_L128;
 }
 
+// The split-K form (automatic choice only): a min/max semiring, the 64x64
+// tiles chosen, a whole number of tiles and of 4-slab groups, every operand
+// 16-byte aligned row by row, fewer tiles than two per CU, no gate.
+static bool splitk_ok(int op, int R, int Cn, int K, const float *X, int64_t ldx, const float *Y,
+                      int64_t ldy, const float *O, int64_t ldo, const Epilogue &e) {
+  if (op == 1 || op == 8 || op < 0 || op > 6 || e.gate) return false;
+  if (R % CfgS::BM || Cn % CfgS::BN || K <= 0 || K % (CfgS::BK * SK_GROUPS)) return false;
+  auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  bool vec = ldx % 4 == 0 && ldy % 4 == 0 && ldo % 4 == 0 && aligned(X) && aligned(Y) &&
+             aligned(O);
+  if (e.C) vec = vec && e.ldc % 4 == 0 && aligned(e.C);
+  if (e.P) vec = vec && e.ldp % 4 == 0 && aligned(e.P);
+  return vec && (int64_t)(R / CfgS::BM) * (Cn / CfgS::BN) < 2 * device_cu_count();
+}
+
 hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,
                            int64_t ldx, const float *Y, int64_t ldy, float *O,
                            int64_t ldo, const Epilogue &e, hipStream_t s,
                            int cfg) {
-  if (cfg == CFG_AUTO) cfg = valu_pick_cfg(op, R, Cn, K);
+  if (cfg == CFG_AUTO) {
+    cfg = valu_pick_cfg(op, R, Cn, K);
+    if (cfg == CFG_S64 && splitk_ok(op, R, Cn, K, X, ldx, Y, ldy, O, ldo, e)) {
+      switch (op) {
+      case 0: return launch_splitk<MinPlus>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+      case 2: return launch_splitk<MaxPlus>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+      case 3: return launch_splitk<MaxMin>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+      case 4: return launch_splitk<MaxMul>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+      case 5: return launch_splitk<MinMul>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+      case 6: return launch_splitk<MinMax>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+      default: break;
+      }
+    }
+  }
   switch (op) {
   case 0: return launch_op<MinPlus>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s, cfg);
   case 1: return launch_op<PlusMul>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s, cfg);

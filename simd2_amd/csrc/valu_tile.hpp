@@ -169,7 +169,9 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
                                               int64_t ldx, const float *__restrict__ Y,
                                               int64_t ldy, int row0, int col0,
                                               const uint16_t *list, int nlist,
-                                              ValuSmem<BM, BN, BK> &sm, float (&acc)[TM][TN]) {
+                                              ValuSmem<BM, BN, BK> &sm, float (&acc)[TM][TN],
+                                              int tid_in = -1, int kt_begin = 0,
+                                              int kt_end = -1) {
   constexpr int NTX = BN / TN;
   constexpr int NT = NTX * (BM / TM);
   constexpr int RS = BM / (TM / 4);
@@ -179,7 +181,10 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
   constexpr int YL = BK * BN / 4 / NT;
   auto &Xs = sm.Xs;
   auto &Ys = sm.Ys;
-  const int tid = tid_x();
+  // tid_in: the thread's index among the NT threads this tile body uses
+  // (default: its workgroup index); [kt_begin, kt_end): the slabs visited
+  // (default: all of K)
+  const int tid = tid_in >= 0 ? tid_in : tid_x();
   const int tx = tid % NTX;
   const int ty = tid / NTX;
 
@@ -189,9 +194,9 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
     for (int j = 0; j < TN; ++j) acc[i][j] = Op::identity;
 
   vf4 xv[XL], yv[YL];
-  const int nkt = LIST ? nlist : (K + BK - 1) / BK;
+  const int nkt = LIST ? nlist : kt_end >= 0 ? kt_end - kt_begin : (K + BK - 1) / BK;
   if (nkt > 0) {
-    const int k0 = LIST ? list[0] * BK : 0;
+    const int k0 = (LIST ? (int)list[0] : kt_begin) * BK;
     load_slab<Op, BM, BN, BK, NT, CHECKED, KT, GUARD>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0,
                                                       col0, k0, tid);
     store_slab<BM, BN, BK, NT, XLD>(Xs[0], Ys[0], xv, yv, tid);
@@ -204,7 +209,7 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
     // written to the idle buffer, which nothing reads afterwards): under a
     // branch the compiler keeps the prefetch in scratch and waits for it.
     const int kn = kt + 1 < nkt ? kt + 1 : kt;
-    const int knext = (LIST ? (int)list[kn] : kn) * BK;
+    const int knext = (LIST ? (int)list[kn] : kt_begin + kn) * BK;
     load_slab<Op, BM, BN, BK, NT, CHECKED, KT, GUARD>(xv, yv, X, ldx, Y, ldy, R, Cn, K, row0,
                                                       col0, knext, tid);
 #pragma unroll
@@ -243,6 +248,12 @@ __device__ __forceinline__ void valu_mainloop(int R, int Cn, int K, const float 
   }
 }
 
+template <class Op, int BM, int BN, int TM, int TN, bool CHECKED, bool MASKED>
+__device__ __forceinline__ void valu_epilogue(int R, int Cn, float *O, int64_t ldo,
+                                              const Epilogue &e, int row0, int col0,
+                                              int *tile_flag, int *changed_slot,
+                                              const float (&acc)[TM][TN], int tid, bool active);
+
 // One BM x BN output tile at (row0, col0) of the row-major problem; the whole
 // workgroup calls it.  tile_flag, when set, receives 1 / 0 for "some element
 // of this tile changed".
@@ -277,8 +288,6 @@ __device__ __forceinline__ void valu_tile(int R, int Cn, int K, const float *__r
   static_assert(NTY * TM == BM && NTX * TN == BN, "thread grid");
 
   const int tid = tid_x();
-  const int tx = tid % NTX;
-  const int ty = tid / NTX;
   if (tid == 0) sm.changed = 0;
 
   float acc[TM][TN];
@@ -288,9 +297,27 @@ __device__ __forceinline__ void valu_tile(int R, int Cn, int K, const float *__r
   (void)XL;
   (void)YL;
   (void)XLD;
+  valu_epilogue<Op, BM, BN, TM, TN, CHECKED, MASKED>(R, Cn, O, ldo, e, row0, col0, tile_flag,
+                                                     &sm.changed, acc, tid, true);
+}
 
-  // Epilogue: (+) C, optional APLP mapping, change test, store.
+// Epilogue of a BM x BN tile from the thread's acc: (+) C, optional APLP
+// mapping, change test, store.  Every thread of the workgroup calls it
+// (the tile flag takes a barrier); only threads with `active` set store, tid
+// being their index among the tile's threads.  *changed_slot (LDS) must be
+// cleared before a barrier that precedes this call.
+template <class Op, int BM, int BN, int TM, int TN, bool CHECKED, bool MASKED>
+__device__ __forceinline__ void valu_epilogue(int R, int Cn, float *O, int64_t ldo,
+                                              const Epilogue &e, int row0, int col0,
+                                              int *tile_flag, int *changed_slot,
+                                              const float (&acc)[TM][TN], int tid, bool active) {
+  constexpr int NTX = BN / TN;
+  constexpr int RS = BM / (TM / 4);
+  constexpr int CS = BN / (TN / 4);
+  const int tx = tid % NTX;
+  const int ty = tid / NTX;
   bool changed = false;
+  if (active) {
 #pragma unroll
   for (int gi = 0; gi < TM / 4; ++gi)
 #pragma unroll
@@ -336,13 +363,14 @@ __device__ __forceinline__ void valu_tile(int R, int Cn, int K, const float *__r
       }
     }
 
+  }
   if (e.flag || tile_flag) {
     const bool any = __any(changed);
     if (e.flag && any && (tid % warpSize) == 0) atomicOr(e.flag, 1);
     if (tile_flag) {
-      if (any && (tid % warpSize) == 0) atomicOr(&sm.changed, 1);
+      if (any && (tid % warpSize) == 0) atomicOr(changed_slot, 1);
       __syncthreads();
-      if (tid == 0) *tile_flag = sm.changed;
+      if (active && tid == 0) *tile_flag = *changed_slot;
     }
   }
 }

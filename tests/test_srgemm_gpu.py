@@ -329,3 +329,30 @@ def test_autotune_records_winner(gpu, oracle_mod):
     want = oracle_mod.srgemm_f32("min_plus", A, B, C, True, threads=8)
     assert np.array_equal(bits(D.cpu().numpy()), bits(want))
     assert simd2_amd.lib().simd2_pick_tile_config(0, N, M, K) == best
+
+
+@pytest.mark.parametrize("M,N,K", [(1024, 1024, 1024), (512, 768, 64), (256, 256, 4096)])
+def test_small_problems_split_k(gpu, oracle_mod, M, N, K):
+    """Problems with fewer 64x64 tiles than two per CU take the split-K form
+    (four groups over contiguous slab ranges, partials combined in order) for
+    the six min/max semirings: the oracle's bits with and without C, and the
+    same bits as the unsplit 64x64 kernel (tile_config 1) on data with signed
+    zeros, whose ties only an order-independent (+) leaves alone."""
+    import simd2_amd
+    for op in EXACT_OPS:
+        for epi in (False, True):
+            A, B, C = semiring_inputs(op, M, N, K, seed=M + 3 * N + K)
+            say(f"split-K {op} {M}x{N}x{K} epi={epi}")
+            want = oracle_mod.srgemm_f32(op, A, B, C, epi, threads=8)
+            assert np.array_equal(bits(run(gpu, op, A, B, C, epi)), bits(want)), (op, epi)
+        rng = np.random.default_rng(M + K)
+        A, B, C = semiring_inputs(op, M, N, K, seed=5)
+        for X in (A, B, C):
+            z = rng.random(X.shape) < 0.3
+            X[z] = np.where(rng.random(int(z.sum())) < 0.5, np.float32(0.0), np.float32(-0.0))
+        tA, tB, tC = (gpu.from_numpy(x).cuda() for x in (A, B, C))
+        say(f"split-K signed zeros {op} {M}x{N}x{K}")
+        Dsplit = simd2_amd.srgemm(op, tA, tB, tC, True)
+        Dseq = simd2_amd.srgemm(op, tA, tB, tC, True, tile_config=1)
+        gpu.cuda.synchronize()
+        assert np.array_equal(bits(Dsplit.cpu().numpy()), bits(Dseq.cpu().numpy())), op
