@@ -59,16 +59,14 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
                                             flag, sm);
 }
 
-// Small problems (fewer 64x64 tiles than two per CU; n = 1024 has 256): the
-// K loop of each tile is split over SK_GROUPS groups of 256 threads in one
-// workgroup, each over a contiguous range of slabs with its own LDS buffers,
-// and the partial tiles are combined in group order, ((p0 (+) p1) (+) p2) (+)
-// p3, before the usual epilogue: four times the resident waves (one per SIMD
-// otherwise).  Only for the semirings whose (+) is min or max, which is
-// exact: regrouping the terms changes no bit.
-constexpr int SK_GROUPS = 4;
-
-template <class Op>
+// Small problems (at most four 64x64 tiles per CU; n = 1024 has one): the K
+// loop of each tile is split over G groups of 256 threads in one workgroup
+// (G = 4 up to two tiles per CU, else 2), each over a contiguous range of
+// slabs with its own LDS buffers, and the partial tiles are combined in
+// group order, ((p0 (+) p1) (+) p2) (+) p3, before the usual epilogue: G
+// times the resident waves.  Only for the semirings whose (+) is min or max,
+// which is exact: regrouping the terms changes no bit.
+template <class Op, int SK_GROUPS>
 __global__ __launch_bounds__(256 * SK_GROUPS) void srgemm_valu_splitk_kernel(
     int R, int Cn, int K, const float *__restrict__ X, int64_t ldx,
     const float *__restrict__ Y, int64_t ldy, float *O, int64_t ldo, Epilogue e,
@@ -114,10 +112,15 @@ __global__ __launch_bounds__(256 * SK_GROUPS) void srgemm_valu_splitk_kernel(
 
 template <class Op>
 hipError_t launch_splitk(int R, int Cn, int K, const float *X, int64_t ldx, const float *Y,
-                         int64_t ldy, float *O, int64_t ldo, const Epilogue &e, hipStream_t s) {
+                         int64_t ldy, float *O, int64_t ldo, const Epilogue &e, int groups,
+                         hipStream_t s) {
   const int tiles_r = R / CfgS::BM, tiles_c = Cn / CfgS::BN;
-  srgemm_valu_splitk_kernel<Op><<<dim3(tiles_r * tiles_c), dim3(256 * SK_GROUPS), 0, s>>>(
-      R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
+  if (groups == 4)
+    srgemm_valu_splitk_kernel<Op, 4><<<dim3(tiles_r * tiles_c), dim3(1024), 0, s>>>(
+        R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
+  else
+    srgemm_valu_splitk_kernel<Op, 2><<<dim3(tiles_r * tiles_c), dim3(512), 0, s>>>(
+        R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
   return hipGetLastError();
 }
 
@@ -224,18 +227,23 @@ int valu_pick_cfg(int op, int R, int Cn, int K) {
 }
 
 // The split-K form (automatic choice only): a min/max semiring, the 64x64
-// tiles chosen, a whole number of tiles and of 4-slab groups, every operand
-// 16-byte aligned row by row, fewer tiles than two per CU, no gate.
-static bool splitk_ok(int op, int R, int Cn, int K, const float *X, int64_t ldx, const float *Y,
-                      int64_t ldy, const float *O, int64_t ldo, const Epilogue &e) {
-  if (op == 1 || op == 8 || op < 0 || op > 6 || e.gate) return false;
-  if (R % CfgS::BM || Cn % CfgS::BN || K <= 0 || K % (CfgS::BK * SK_GROUPS)) return false;
+// tiles chosen, a whole number of tiles and of G-slab groups, every operand
+// 16-byte aligned row by row, at most four tiles per CU, no gate.  Returns
+// the group count G, 0 when the form does not apply.
+static int splitk_groups(int op, int R, int Cn, int K, const float *X, int64_t ldx,
+                         const float *Y, int64_t ldy, const float *O, int64_t ldo,
+                         const Epilogue &e) {
+  if (op == 1 || op == 8 || op < 0 || op > 6 || e.gate) return 0;
+  if (R % CfgS::BM || Cn % CfgS::BN || K <= 0) return 0;
+  const int64_t tiles = (int64_t)(R / CfgS::BM) * (Cn / CfgS::BN);
+  const int groups = tiles < 2 * device_cu_count() ? 4 : tiles <= 4 * device_cu_count() ? 2 : 0;
+  if (groups == 0 || K % (CfgS::BK * groups)) return 0;
   auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
   bool vec = ldx % 4 == 0 && ldy % 4 == 0 && ldo % 4 == 0 && aligned(X) && aligned(Y) &&
              aligned(O);
   if (e.C) vec = vec && e.ldc % 4 == 0 && aligned(e.C);
   if (e.P) vec = vec && e.ldp % 4 == 0 && aligned(e.P);
-  return vec && (int64_t)(R / CfgS::BM) * (Cn / CfgS::BN) < 2 * device_cu_count();
+  return vec ? groups : 0;
 }
 
 hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,
@@ -244,14 +252,15 @@ hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,
                            int cfg) {
   if (cfg == CFG_AUTO) {
     cfg = valu_pick_cfg(op, R, Cn, K);
-    if (cfg == CFG_S64 && splitk_ok(op, R, Cn, K, X, ldx, Y, ldy, O, ldo, e)) {
+    const int g = cfg == CFG_S64 ? splitk_groups(op, R, Cn, K, X, ldx, Y, ldy, O, ldo, e) : 0;
+    if (g) {
       switch (op) {
-      case 0: return launch_splitk<MinPlus>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
-      case 2: return launch_splitk<MaxPlus>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
-      case 3: return launch_splitk<MaxMin>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
-      case 4: return launch_splitk<MaxMul>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
-      case 5: return launch_splitk<MinMul>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
-      case 6: return launch_splitk<MinMax>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
+      case 0: return launch_splitk<MinPlus>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, g, s);
+      case 2: return launch_splitk<MaxPlus>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, g, s);
+      case 3: return launch_splitk<MaxMin>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, g, s);
+      case 4: return launch_splitk<MaxMul>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, g, s);
+      case 5: return launch_splitk<MinMul>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, g, s);
+      case 6: return launch_splitk<MinMax>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, g, s);
       default: break;
       }
     }

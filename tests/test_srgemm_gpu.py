@@ -331,10 +331,12 @@ def test_autotune_records_winner(gpu, oracle_mod):
     assert simd2_amd.lib().simd2_pick_tile_config(0, N, M, K) == best
 
 
-@pytest.mark.parametrize("M,N,K", [(1024, 1024, 1024), (512, 768, 64), (256, 256, 4096)])
+@pytest.mark.parametrize("M,N,K", [(1024, 1024, 1024), (512, 768, 64), (256, 256, 4096),
+                                   (1536, 1536, 256), (2048, 2048, 64)])
 def test_small_problems_split_k(gpu, oracle_mod, M, N, K):
-    """Problems with fewer 64x64 tiles than two per CU take the split-K form
-    (four groups over contiguous slab ranges, partials combined in order) for
+    """Problems with at most four 64x64 tiles per CU take the split-K form
+    (four groups over contiguous slab ranges up to two tiles per CU, two
+    groups up to four: 1536^2 and 2048^2; partials combined in order) for
     the six min/max semirings: the oracle's bits with and without C, and the
     same bits as the unsplit 64x64 kernel (tile_config 1) on data with signed
     zeros, whose ties only an order-independent (+) leaves alone."""
