@@ -214,6 +214,21 @@ hipError_t launch_sched_occupancy(int op, int cfg, const float *D, int64_t ld, i
                                   int tiles_r, int tiles_c, int slabs, uint8_t *xocc,
                                   uint8_t *yocc, const SchedArgs *a, hipStream_t s);
 hipError_t launch_fill_f32(float *p, float v, int64_t n, hipStream_t s);
+// Up to 8 fills of 32-bit words in one launch (one kernel instead of one per
+// buffer: for small problems the launches, not the stores, take the time).
+struct FillSegments {
+  uint32_t *p[8];
+  int64_t n[8];
+  uint32_t v[8];
+  int k = 0;
+  void add(void *ptr, int64_t count, uint32_t value) {
+    p[k] = static_cast<uint32_t *>(ptr);
+    n[k] = count;
+    v[k] = value;
+    ++k;
+  }
+};
+hipError_t launch_fill_multi(const FillSegments &f, hipStream_t s);
 hipError_t launch_permute_copy(float *dst, int64_t ldd, const float *src, int64_t lds, int n,
                                const int *perm, int inverse, hipStream_t s);
 

@@ -27,17 +27,28 @@ namespace simd2 {
 namespace {
 
 constexpr int EW = 256;
+constexpr int SCHED_SPLIT_SLABS = 256;  // the split form's limit: n <= 4096
 
-template <class Op, class Cfg>
-__global__ __launch_bounds__((Cfg::BM / Cfg::TM) * (Cfg::BN / Cfg::TN))
-__attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) {
+// G > 1 (small problems, 64x64 tiles): the tile's occupied slabs are split
+// over G groups of 256 threads, contiguous ranges of the list, and the
+// partial tiles combined in group order before the epilogue, as the plain
+// kernel's split-K form does (min/max: exact, so regrouping changes no bit).
+// Every group runs the same number of slabs: a shorter range repeats its
+// last slab (or the list's first), which min/max, being idempotent, ignore.
+template <class Op, class Cfg, int G = 1>
+__global__ __launch_bounds__((Cfg::BM / Cfg::TM) * (Cfg::BN / Cfg::TN) * G)
+__attribute__((amdgpu_waves_per_eu(G > 1 ? 1 : Cfg::WAVES))) void sched_kernel(SchedArgs a) {
   constexpr int BM = Cfg::BM, BN = Cfg::BN, TM = Cfg::TM, TN = Cfg::TN, BK = Cfg::BK;
   constexpr int NTX = BN / TN;
+  constexpr int NT = NTX * (BM / TM);  // threads of one group
   constexpr int RS = BM / (TM / 4), CS = BN / (TN / 4);
   constexpr int XS = BN / 16;  // column slabs of a tile
   constexpr int YS = BM / 16;  // row slabs of a tile
-  static_assert(BK == 16 && XS <= 32 && YS <= 32 && XS + YS <= 256, "slab geometry");
-  __shared__ ValuSmem<BM, BN, BK> sm;
+  static_assert(BK == 16 && XS <= 32 && YS <= 32 && XS + YS <= 256 && NT == 256,
+                "slab geometry");
+  constexpr int GL = G > 1 ? SCHED_SPLIT_SLABS / G : 1;  // slabs per group (launcher checks)
+  __shared__ ValuSmem<BM, BN, BK> sm_g[G];
+  __shared__ uint16_t glist[G][GL];
   __shared__ uint16_t list[SCHED_MAX_SLABS];
   __shared__ int s_cnt;
   __shared__ int s_wave[4];
@@ -47,14 +58,17 @@ __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) 
   // stays where that squaring left it.
   if (a.conv_prev && *a.conv_prev == 0) return;
   if (a.sq_mode && *a.sq_mode != 0) return;  // squared row by row instead
-  const int tid = tid_x();
+  const int tid_all = tid_x();
+  const int grp = tid_all / NT, tid = tid_all % NT;
+  auto &sm = sm_g[grp];
   int tr, tc;
   map_tile(bid_x(), a.tiles_r, a.tiles_c, tr, tc);
   if (a.rowchg_prev && !a.rowchg_prev[tr] && !a.colchg_prev[tc]) return;  // clean tile
   const int row0 = tr * BM, col0 = tc * BN;
 
   // Occupied slabs of this tile, ascending.
-  if (tid == 0) {
+  // (group 0 builds the list; the other groups only keep the barriers)
+  if (tid_all == 0) {
     s_cnt = 0;
     s_bits[0] = 0;
     s_bits[1] = 0;
@@ -65,22 +79,58 @@ __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) 
   const int lane = tid % 64, wave = tid / 64;
   for (int base = 0; base < a.slabs; base += 256) {
     const int s = base + tid;
-    const bool p = s < a.slabs && xo[s] && yo[s];
+    const bool p = grp == 0 && s < a.slabs && xo[s] && yo[s];
     const uint64_t m = __ballot(p);
-    if (lane == 0) s_wave[wave] = __popcll(m);
+    if (grp == 0 && lane == 0) s_wave[wave] = __popcll(m);
     __syncthreads();
-    int off = s_cnt;
-    for (int w = 0; w < wave; ++w) off += s_wave[w];
-    if (p) list[off + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)s;
+    if (grp == 0) {
+      int off = s_cnt;
+      for (int w = 0; w < wave; ++w) off += s_wave[w];
+      if (p) list[off + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)s;
+    }
     __syncthreads();
-    if (tid == 0) s_cnt += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    if (tid_all == 0) s_cnt += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
     __syncthreads();
   }
   const int nlist = s_cnt;
 
   float acc[TM][TN];
-  valu_mainloop<Op, BM, BN, TM, TN, BK, false, true>(a.n, a.n, a.kpad, a.cur, a.ld, a.cur, a.ld,
-                                                     row0, col0, list, nlist, sm, acc);
+  if constexpr (G == 1) {
+    valu_mainloop<Op, BM, BN, TM, TN, BK, false, true>(a.n, a.n, a.kpad, a.cur, a.ld, a.cur,
+                                                       a.ld, row0, col0, list, nlist, sm, acc);
+  } else {
+    // group g: entries [b, b + c) of the list, padded to `steps` by repeats
+    const int steps = (nlist + G - 1) / G;
+    const int b = grp * (nlist / G) + min(grp, nlist % G);
+    const int c = nlist / G + (grp < nlist % G ? 1 : 0);
+    for (int i = tid; i < steps; i += NT)
+      glist[grp][i] = list[c > 0 ? b + min(i, c - 1) : 0];
+    __syncthreads();
+    valu_mainloop<Op, BM, BN, TM, TN, BK, false, true>(a.n, a.n, a.kpad, a.cur, a.ld, a.cur,
+                                                       a.ld, row0, col0, glist[grp], steps, sm,
+                                                       acc, tid);
+    // the main loop ends with a barrier after every read of the slabs
+    if (grp > 0) {
+      float *part = reinterpret_cast<float *>(&sm);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) part[(i * TN + j) * NT + tid] = acc[i][j];
+    }
+    __syncthreads();
+    if (grp == 0) {
+#pragma unroll
+      for (int q = 1; q < G; ++q) {
+        const float *part = reinterpret_cast<const float *>(&sm_g[q]);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = Op::add(acc[i][j], part[(i * TN + j) * NT + tid]);
+      }
+    }
+  }
+  const bool active = grp == 0;
 
   // Epilogue.  The padded layout keeps the 128-bit reads of the old values in
   // bounds; stores and flags cover only r < n, c < n.
@@ -88,6 +138,7 @@ __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) 
   const uint32_t ident = __float_as_uint(Op::identity);
   bool chg_bits = false, chg_test = false;
   unsigned xb = 0, yb = 0;
+  if (active)
 #pragma unroll
   for (int gi = 0; gi < TM / 4; ++gi)
 #pragma unroll
@@ -142,12 +193,12 @@ __attribute__((amdgpu_waves_per_eu(Cfg::WAVES))) void sched_kernel(SchedArgs a) 
   if (__any(chg_test) && lane == 0) atomicOr(a.conv, 1);
   __syncthreads();
   const int sx = col0 / 16 + tid, sy = row0 / 16 + (tid - XS);
-  if (tid < XS) {
+  if (active && tid < XS) {
     if (sx < a.slabs) a.xocc_out[(int64_t)tr * a.slabs + sx] = (s_bits[0] >> tid) & 1u;
-  } else if (tid < XS + YS) {
+  } else if (active && tid < XS + YS) {
     if (sy < a.slabs) a.yocc_out[(int64_t)tc * a.slabs + sy] = (s_bits[1] >> (tid - XS)) & 1u;
   }
-  if (tid == 0 && a.stats) {
+  if (tid_all == 0 && a.stats) {
     atomicAdd(a.stats, 1ull);
     atomicAdd(a.stats + 1, (unsigned long long)nlist);
   }
@@ -628,6 +679,15 @@ __global__ __launch_bounds__(EW) void fill_f32_kernel(float *p, float v, int64_t
   for (int64_t i = (int64_t)bid_x() * EW + tid_x(); i < n; i += (int64_t)gridDim.x * EW) p[i] = v;
 }
 
+// Several 32-bit fills in one launch: segment blockIdx.y.
+__global__ __launch_bounds__(EW) void fill_multi_kernel(FillSegments f) {
+  const int k = bid_y();
+  uint32_t *p = f.p[k];
+  const int64_t n = f.n[k];
+  for (int64_t i = (int64_t)bid_x() * EW + tid_x(); i < n; i += (int64_t)gridDim.x * EW)
+    p[i] = f.v[k];
+}
+
 // inverse == 0: dst[i][j] = src[perm[i]][perm[j]]; inverse == 1:
 // dst[perm[i]][perm[j]] = src[i][j].  perm == null: plain copy.
 __global__ __launch_bounds__(EW) void permute_copy_kernel(float *dst, int64_t ldd,
@@ -645,6 +705,19 @@ __global__ __launch_bounds__(EW) void permute_copy_kernel(float *dst, int64_t ld
 }
 
 int grid_for(int64_t n) { return (int)std::min<int64_t>((n + EW - 1) / EW, 4096); }
+
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+    else
+      cus = 256;
+  }
+  return cus;
+}
 
 template <class Op>
 hipError_t sched_op(int cfg, const SchedArgs &a, hipStream_t s) {
@@ -665,7 +738,14 @@ hipError_t sched_op(int cfg, const SchedArgs &a, hipStream_t s) {
       symmetrize_kernel<Op><<<dim3(T * (T + 1) / 2), dim3(EW), 0, s>>>(a, c.bm, c.bn);
     }
   }
-  if (cfg == CFG_S64)
+  // at most four 64x64 tiles per CU: the tile's slabs split over 4 or 2 groups
+  const int tiles = a.tiles_r * a.tiles_c, cus = device_cus();
+  const bool split = cfg == CFG_S64 && a.slabs <= SCHED_SPLIT_SLABS;
+  if (split && tiles < 2 * cus)
+    sched_kernel<Op, CfgS, 4><<<grid, dim3(1024), 0, s>>>(a);
+  else if (split && tiles <= 4 * cus)
+    sched_kernel<Op, CfgS, 2><<<grid, dim3(512), 0, s>>>(a);
+  else if (cfg == CFG_S64)
     sched_kernel<Op, CfgS><<<grid, block, 0, s>>>(a);
   else if (cfg == CFG_M128x64)
     sched_kernel<Op, CfgM><<<grid, block, 0, s>>>(a);
@@ -703,6 +783,15 @@ hipError_t launch_sched_symmetry(const float *D, int64_t ld, int n, int *asym, h
   if (n <= 0) return hipSuccess;
   const int T = (n + SY - 1) / SY;
   symmetry_check_kernel<<<dim3(T * (T + 1) / 2), dim3(EW), 0, s>>>(D, ld, n, asym);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_multi(const FillSegments &f, hipStream_t s) {
+  if (f.k <= 0) return hipSuccess;
+  int64_t most = 0;
+  for (int i = 0; i < f.k; ++i) most = std::max(most, f.n[i]);
+  if (most <= 0) return hipSuccess;
+  fill_multi_kernel<<<dim3(grid_for(most), f.k), dim3(EW), 0, s>>>(f);
   return hipGetLastError();
 }
 

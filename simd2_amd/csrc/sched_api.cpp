@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cstdint>
 #include <vector>
 
@@ -159,21 +160,25 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   const float ident = (op == SIMD2_MIN_PLUS || op == SIMD2_MIN_MAX) ? inf : -inf;
 
   // Padded copies: everything outside the live n x n block is the identity.
+  // (the identity in both buffers and every counter cleared: one launch)
   const int64_t P2 = (int64_t)L.P * L.P;
-  hipError_t e = launch_fill_f32(buf[0], ident, P2, stream);
-  if (e == hipSuccess) e = launch_fill_f32(buf[1], ident, P2, stream);
+  uint32_t ident_bits;
+  std::memcpy(&ident_bits, &ident, 4);
+  FillSegments fills;
+  fills.add(buf[0], P2, ident_bits);
+  fills.add(buf[1], P2, ident_bits);
+  fills.add(rowchg, (int64_t)(max_itr + 1) * L.tiles_r, 0);
+  fills.add(colchg, (int64_t)(max_itr + 1) * L.tiles_c, 0);
+  fills.add(conv, max_itr + 1, 0);
+  fills.add(work, 2 * (int64_t)(max_itr + 1), 0);
+  fills.add(mode, max_itr + 1, 0);
+  fills.add(stats, 12, 0);  // 6 64-bit counters
+  hipError_t e = launch_fill_multi(fills, stream);
   if (e == hipSuccess) e = launch_permute_copy(buf[0], L.P, X, ldx, n, perm, 0, stream);
-  if (e == hipSuccess) e = launch_fill_i32(rowchg, 0, (int64_t)(max_itr + 1) * L.tiles_r, stream);
-  if (e == hipSuccess) e = launch_fill_i32(colchg, 0, (int64_t)(max_itr + 1) * L.tiles_c, stream);
-  if (e == hipSuccess) e = launch_fill_i32(conv, 0, max_itr + 1, stream);
-  if (e == hipSuccess)
-    e = launch_fill_i32(reinterpret_cast<int *>(work), 0, 2 * (int64_t)(max_itr + 1), stream);
-  if (e == hipSuccess) e = launch_fill_i32(mode, 0, max_itr + 1, stream);
   // a symmetric input stays symmetric (every (x) here commutes bit for bit),
   // which the changed-entry form can use
   if (e == hipSuccess && changed_on) e = launch_fill_i32(asym, 0, 1, stream);
   if (e == hipSuccess && changed_on) e = launch_sched_symmetry(buf[0], L.P, n, asym, stream);
-  if (e == hipSuccess) e = launch_fill_i32(reinterpret_cast<int *>(stats), 0, 12, stream);
   if (e == hipSuccess)
     e = launch_sched_occupancy(op, cfg, buf[0], L.P, n, L.tiles_r, L.tiles_c, L.slabs, xocc[0],
                                yocc[0], nullptr, stream);
