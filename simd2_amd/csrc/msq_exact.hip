@@ -78,6 +78,7 @@ __device__ __forceinline__ bool small_int(float v) {
 // [n, Rq) are written as zeros; norms only for rows < n.  Every load of a
 // step is issued before any is used (addresses clamped into the matrix,
 // values outside it replaced by 0), so a step waits for memory once.
+template <bool VEC>
 __global__ __launch_bounds__(256) void msq_quantize_kernel(
     int R, int Cn, int K, int Kp, const float *__restrict__ X, int64_t ldx,
     const float *__restrict__ Y, int64_t ldy, int8_t *__restrict__ Xq,
@@ -96,7 +97,23 @@ __global__ __launch_bounds__(256) void msq_quantize_kernel(
   uint32_t badmask = 0;  // X: bit j = row rg + 4j; Y: bit 0 = column cc
   constexpr int NV = QR * QS / 256;
   auto issue = [&](int k0, float (&v)[NV]) {
-    if (isx) {
+    if (isx && VEC) {
+      // X[i][k], k contiguous: lane l reads k0 + 4l .. +3 of a row, 16 bytes
+      // (a wave: 1 KB of one row); past K element by element, clamped
+      const int l = tid % 64, rg = tid / 64, k = k0 + 4 * l;
+#pragma unroll
+      for (int j = 0; j < QR / 4; ++j) {
+        const float *row = X + (int64_t)min(i0 + rg + 4 * j, n - 1) * ldx;
+        if (k + 3 < K) {
+          const f32x4 q4 = *reinterpret_cast<const f32x4 *>(row + k);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v[j * 4 + q] = q4[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v[j * 4 + q] = row[min(k + q, K - 1)];
+        }
+      }
+    } else if (isx) {
       // X[i][k], k contiguous: a wave reads 64 consecutive k of one row
       const int kk = tid % 64, rg = tid / 64;
 #pragma unroll
@@ -107,15 +124,39 @@ __global__ __launch_bounds__(256) void msq_quantize_kernel(
           v[j * (QS / 64) + s] = X[(int64_t)i * ldx + min(k0 + 64 * s + kk, K - 1)];
       }
     } else {
-      // Y[k][c], c contiguous: half a wave reads 32 consecutive columns
-      const int cc = tid % QR, kr = tid / QR;
+      // Y[k][c], c contiguous: half a wave reads 32 consecutive columns; the
+      // thread takes 4 consecutive k of its column at a time (one LDS word)
+      const int cc = tid % QR, kq = tid / QR;
       const int c = min(i0 + cc, n - 1);
 #pragma unroll
-      for (int j = 0; j < NV; ++j) v[j] = Y[(int64_t)min(k0 + kr + 8 * j, K - 1) * ldy + c];
+      for (int j = 0; j < NV / 4; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          v[j * 4 + q] = Y[(int64_t)min(k0 + 4 * (kq + 8 * j) + q, K - 1) * ldy + c];
     }
   };
+  // 4 consecutive k of one row or column, as int8 in one 32-bit word
+  auto pack4 = [&](const float (&v)[NV], int at, int k, bool row_ok, uint32_t bad_bit) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = (row_ok && k + q < K) ? v[at + q] : 0.0f;
+      const bool ok = small_int(x);
+      badmask |= ok ? 0u : bad_bit;
+      w |= (uint32_t)(uint8_t)(ok ? (int8_t)(int)x : (int8_t)0) << (8 * q);
+    }
+    return w;
+  };
   auto consume = [&](int k0, const float (&v)[NV]) {
-    if (isx) {
+    if (isx && VEC) {
+      const int l = tid % 64, rg = tid / 64;
+#pragma unroll
+      for (int j = 0; j < QR / 4; ++j) {
+        const int ri = rg + 4 * j;
+        *reinterpret_cast<uint32_t *>(&T[ri][4 * l]) =
+            pack4(v, j * 4, k0 + 4 * l, i0 + ri < n, 1u << j);
+      }
+    } else if (isx) {
       const int kk = tid % 64, rg = tid / 64;
 #pragma unroll
       for (int j = 0; j < QR / 4; ++j) {
@@ -130,14 +171,12 @@ __global__ __launch_bounds__(256) void msq_quantize_kernel(
         }
       }
     } else {
-      const int cc = tid % QR, kr = tid / QR;
+      const int cc = tid % QR, kq = tid / QR;
 #pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        const int k = k0 + kr + 8 * j;
-        const float y = (i0 + cc < n && k < K) ? v[j] : 0.0f;
-        const bool ok = small_int(y);
-        badmask |= ok ? 0u : 1u;
-        T[cc][kr + 8 * j] = ok ? (int8_t)(int)y : (int8_t)0;
+      for (int j = 0; j < NV / 4; ++j) {
+        const int m = kq + 8 * j;
+        *reinterpret_cast<uint32_t *>(&T[cc][4 * m]) =
+            pack4(v, j * 4, k0 + 4 * m, i0 + cc < n, 1u);
       }
     }
     __syncthreads();
@@ -551,8 +590,13 @@ hipError_t launch_minus_square_exact(int R, int Cn, int K, const float *X, int64
   hipError_t e = launch_fill_i32(nx, 0, R, s);
   if (e == hipSuccess) e = launch_fill_i32(ny, 0, Cn, s);
   if (e != hipSuccess) return e;
-  msq_quantize_kernel<<<dim3(qbx + qby, ksplit), 256, 0, s>>>(R, Cn, K, w.Kp, X, ldx, Y, ldy, Xq,
-                                                             Yq, nx, ny, qbx, kchunk);
+  // 128-bit loads of X rows when they are 16-byte aligned
+  if (ldx % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0)
+    msq_quantize_kernel<true><<<dim3(qbx + qby, ksplit), 256, 0, s>>>(
+        R, Cn, K, w.Kp, X, ldx, Y, ldy, Xq, Yq, nx, ny, qbx, kchunk);
+  else
+    msq_quantize_kernel<false><<<dim3(qbx + qby, ksplit), 256, 0, s>>>(
+        R, Cn, K, w.Kp, X, ldx, Y, ldy, Xq, Yq, nx, ny, qbx, kchunk);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int tiles_r = w.Rq / TR, tiles_c = w.Cq / TC;
