@@ -174,3 +174,22 @@ def test_exact_check_mode_every_form(gpu, op):
             Z, itr2, _, rep = converge_scheduled(op, X.clone(), check="exact", row_sparse=rs,
                                                  changed_sparse=cs, predicted_itr=3)
             assert itr2 == itr and torch.equal(Z.view(torch.int32), Y.view(torch.int32)), rep
+
+
+@pytest.mark.parametrize("kind", ["er_dense", "grid"])
+def test_two_group_split_matches_plain_loop(gpu, kind):
+    """n = 1600: 625 tiles of 64x64, between two and four per CU, so each
+    scheduled squaring splits a tile's slab list over two thread groups (the
+    n <= 640 cases above split over four); same matrix and count as the plain
+    loop."""
+    torch = gpu
+    n = 1600
+    for k, (op, epi, check) in enumerate(OPS[:2]):
+        X = family(torch, kind, op, n, 7 * n + k)
+        Y, itr, st = plain(torch, op, X, epi, check)
+        assert st == simd2_amd.SUCCESS
+        print(f"{kind} n={n} {op} two-group split", flush=True)
+        Z, itr2, st2, rep = converge_scheduled(op, X.clone(), epilogue=epi, check=check,
+                                               tile_config=1)
+        assert st2 == simd2_amd.SUCCESS and itr2 == itr, (itr2, itr)
+        assert torch.equal(Z.view(torch.int32), Y.view(torch.int32))
