@@ -33,6 +33,7 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct Launch {
   int op, cfg;
+  int branch;  // parallel branch of the captured round that runs it
   int items;  // work items (tiles) of the launch
   int grid;   // workgroups: 8 x the longest per-XCD list
 };
@@ -70,6 +71,19 @@ struct simd2_batch {
 namespace {
 
 bool is_float_op(int op) { return op != SIMD2_OR_AND; }
+
+// Time of one semiring term relative to min_plus's in the VALU kernels (the
+// single-problem kernels' measured rates at 4096^3); or_and's terms are bits.
+double op_weight(int op) {
+  switch (op) {
+    case SIMD2_MAX_MIN:
+    case SIMD2_MIN_MAX: return 1.6;
+    case SIMD2_MAX_MUL:
+    case SIMD2_MIN_MUL: return 1.15;
+    case SIMD2_OR_AND: return 0.07;
+    default: return 1.0;
+  }
+}
 
 // Work of one tile row (or word block) of graph g under `cfg`: proportional
 // to the tile area the kernel sweeps times K.
@@ -119,9 +133,10 @@ BatchLaunchDev build_items(const simd2_batch &b, const std::vector<int> &gs, int
   return L;
 }
 
-// Groups graphs by (op, cfg) into launches (largest total work first), fills
-// `items` and the launch descriptors, and records each graph's launch in
-// cls[g].  FP32 graphs with cfg < 0 are left out (the autotuner plans one size
+// Groups graphs by (op, cfg) into launches (largest estimated time first),
+// fills `items` and the launch descriptors, and records each graph's launch
+// in cls[g].  Launches go to the round's parallel branches longest first,
+// each to the branch with the least estimated time so far.  FP32 graphs with cfg < 0 are left out (the autotuner plans one size
 // bin at a time), or_and graphs unless with_orand.
 void plan(const simd2_batch &b, const std::vector<int> &cfg, bool with_orand,
           std::vector<Launch> &launches, std::vector<BatchLaunchDev> &ldev,
@@ -139,17 +154,22 @@ void plan(const simd2_batch &b, const std::vector<int> &cfg, bool with_orand,
       int tr, tc;
       w += tiles_of(b, g, kv.first.second, &tr, &tc) * tile_cost(b, g, kv.first.second);
     }
-    order.push_back({-w, kv.first});
+    order.push_back({-w * op_weight(kv.first.first), kv.first});
   }
   std::sort(order.begin(), order.end());
   launches.clear();
   ldev.clear();
   items.clear();
   cls.assign(b.count, 0);
+  const int nb = std::max(1, std::min<int>(BRANCHES, (int)order.size()));
+  std::vector<double> branch_load(nb, 0.0);
   for (auto &o : order) {
     Launch L;
     L.op = o.second.first;
     L.cfg = o.second.second;
+    L.branch = (int)(std::min_element(branch_load.begin(), branch_load.end()) -
+                     branch_load.begin());
+    branch_load[L.branch] -= o.first;
     const int first = (int)items.size();
     for (int g : classes[o.second]) cls[g] = (int)launches.size();
     const BatchLaunchDev d = build_items(b, classes[o.second], L.cfg, items);
@@ -218,8 +238,8 @@ hipError_t launch_round(simd2_batch &b, int r, hipStream_t s) {
 }
 
 // Round r with its class launches as parallel branches: the class counts are
-// cleared on s, then launch i runs on branch stream i % nb after an event of
-// s, and s waits for every branch before the round's update.  Graphs of
+// cleared on s, then each launch runs on its branch's stream after an event
+// of s, and s waits for every branch before the round's update.  Graphs of
 // different classes share no memory, so the classes may overlap: the tail of
 // one launch fills with the next one's workgroups, and in the late rounds the
 // classes whose graphs take more squarings run side by side.  Used while s
@@ -244,7 +264,7 @@ hipError_t launch_round_branched(simd2_batch &b, int r, hipStream_t s,
     const Launch &L = b.launches[i];
     e = launch_batch_round(L.op, L.cfg, L.grid, b.d_items, b.d_launch + i, b.d_desc, b.pool,
                            b.active(), b.changed(), r & 1,
-                           r > 0 ? b.d_class + (int64_t)r * nl + i : nullptr, br[i % nb]);
+                           r > 0 ? b.d_class + (int64_t)r * nl + i : nullptr, br[L.branch % nb]);
   }
   for (int k = 0; k < nb && e == hipSuccess; ++k) {
     hipEvent_t join;
