@@ -237,12 +237,14 @@ __device__ void bfs(const uint32_t *adj, int n, int nw, int src, int *cur, int *
 // word t % 32 for every eighth frontier vertex); wave 0 then takes the fresh
 // bits, numbers them by a prefix sum over the words and writes the next
 // list.  Same results as bfs(): depth, reached count, lowest vertex of the
-// last level, levels.
+// last level, levels.  The LDS (136 KB) allows one workgroup per CU, so it
+// takes 1024 threads: 32 frontier vertices expanded at once.
 constexpr int SB_NW = 32;
 constexpr int SB_N = SB_NW * 32;
+constexpr int SB_T = 1024;
 
 struct SmallBfsSmem {
-  uint32_t adj[SB_N * SB_NW];
+  alignas(16) uint32_t adj[SB_N * SB_NW];
   int list[2][SB_N];
   uint32_t visited[SB_NW];
   uint32_t nextb[SB_NW];
@@ -250,20 +252,29 @@ struct SmallBfsSmem {
 };
 
 __device__ void load_adj_small(const uint32_t *adj, int n, int nw, SmallBfsSmem &sm) {
-  for (int i = tid_x(); i < n * nw; i += BFS_T) sm.adj[i] = adj[i];
+  const int words = n * nw;
+  int done = 0;
+  if ((reinterpret_cast<uintptr_t>(adj) & 15) == 0) {  // 16-byte copies
+    const int w4 = words / 4;
+    const uint4 *src = reinterpret_cast<const uint4 *>(adj);
+    uint4 *dst = reinterpret_cast<uint4 *>(sm.adj);
+    for (int i = tid_x(); i < w4; i += SB_T) dst[i] = src[i];
+    done = 4 * w4;
+  }
+  for (int i = done + tid_x(); i < words; i += SB_T) sm.adj[i] = adj[i];
   __syncthreads();
 }
 
 __device__ void bfs_small(int n, int nw, int src, int *level, int *res, SmallBfsSmem &sm) {
   const int t = tid_x();
-  constexpr int G = BFS_T / SB_NW;
+  constexpr int G = SB_T / SB_NW;
   const int g = t / SB_NW, w = t % SB_NW;
   if (t < SB_NW) {
     sm.visited[t] = 0;
     sm.nextb[t] = 0;
   }
   if (level)
-    for (int v = t; v < n; v += BFS_T) level[v] = v == src ? 0 : -1;
+    for (int v = t; v < n; v += SB_T) level[v] = v == src ? 0 : -1;
   __syncthreads();
   if (t == 0) {
     sm.visited[src / 32] |= 1u << (src % 32);
@@ -322,7 +333,7 @@ __device__ void bfs_small(int n, int nw, int src, int *level, int *res, SmallBfs
   __syncthreads();
 }
 
-__global__ __launch_bounds__(BFS_T) void bfs_sources_small_kernel(const uint32_t *adj, int n,
+__global__ __launch_bounds__(SB_T) void bfs_sources_small_kernel(const uint32_t *adj, int n,
                                                                   int nw, int *res) {
   __shared__ SmallBfsSmem sm;
   load_adj_small(adj, n, nw, sm);
@@ -330,7 +341,7 @@ __global__ __launch_bounds__(BFS_T) void bfs_sources_small_kernel(const uint32_t
   bfs_small(n, nw, src, nullptr, res + 3 * bid_x(), sm);
 }
 
-__global__ __launch_bounds__(BFS_T) void bfs_levels_small_kernel(const uint32_t *adj, int n,
+__global__ __launch_bounds__(SB_T) void bfs_levels_small_kernel(const uint32_t *adj, int n,
                                                                  int nw, int *level, int *res,
                                                                  const int *far) {
   __shared__ SmallBfsSmem sm;
@@ -659,7 +670,7 @@ hipError_t launch_graph_features(int op, int n, const float *X, int64_t ld, int 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (nw <= SB_NW)
-    bfs_sources_small_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, res);
+    bfs_sources_small_kernel<<<dim3(NSRC), dim3(SB_T), 0, s>>>(adj, n, nw, res);
   else
     bfs_sources_kernel<<<dim3(NSRC), dim3(BFS_T), 0, s>>>(adj, n, nw, lists, res);
   const int max_based = (op == 2 || op == 3 || op == 4) ? 1 : 0;
@@ -683,7 +694,7 @@ hipError_t launch_graph_levels(int n, const float *X, int64_t ld, int *level, vo
   }
   const int nw = (n + 31) / 32;
   if (nw <= SB_NW)
-    bfs_levels_small_kernel<<<dim3(1), dim3(BFS_T), 0, s>>>(adj, n, nw, level, res + 3 * NSRC,
+    bfs_levels_small_kernel<<<dim3(1), dim3(SB_T), 0, s>>>(adj, n, nw, level, res + 3 * NSRC,
                                                             after_features ? res + 2 : nullptr);
   else
     bfs_levels_kernel<<<dim3(1), dim3(BFS_T), 0, s>>>(adj, n, nw, lists, level, res + 3 * NSRC,

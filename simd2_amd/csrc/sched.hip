@@ -653,12 +653,22 @@ __global__ __launch_bounds__(EW) void occupancy_kernel(const float *D, int64_t l
     s_bits[1] = 0;
   }
   __syncthreads();
+  // Each thread reads one 16-byte column group of every (EW / (bn/4))-th
+  // tile row: rows are 16-byte aligned (ld is a multiple of 128 floats) and
+  // a row's padded columns up to the tile edge lie inside its pitch.
+  const int vpr = bn / 4;
+  const int c4 = tid % vpr;
+  const int c = col0 + 4 * c4;
   unsigned xb = 0, yb = 0;
-  for (int e = tid; e < bm * bn; e += EW) {
-    const int rl = e / bn, cl = e % bn;
-    const int r = row0 + rl, c = col0 + cl;
-    if (r < n && c < n && __float_as_uint(D[(int64_t)r * ld + c]) != ident) {
-      xb |= 1u << (cl / 16);
+#pragma unroll 4
+  for (int rl = tid / vpr; rl < bm; rl += EW / vpr) {
+    const int r = row0 + rl;
+    if (r >= n) break;
+    const uint4 v = *reinterpret_cast<const uint4 *>(D + (int64_t)r * ld + c);
+    const bool any = (c < n && v.x != ident) || (c + 1 < n && v.y != ident) ||
+                     (c + 2 < n && v.z != ident) || (c + 3 < n && v.w != ident);
+    if (any) {
+      xb |= 1u << (c4 / 4);
       yb |= 1u << (rl / 16);
     }
   }
@@ -679,13 +689,22 @@ __global__ __launch_bounds__(EW) void fill_f32_kernel(float *p, float v, int64_t
   for (int64_t i = (int64_t)bid_x() * EW + tid_x(); i < n; i += (int64_t)gridDim.x * EW) p[i] = v;
 }
 
-// Several 32-bit fills in one launch: segment blockIdx.y.
+// Several 32-bit fills in one launch: segment blockIdx.y, 16-byte stores
+// from a 16-byte aligned start.
 __global__ __launch_bounds__(EW) void fill_multi_kernel(FillSegments f) {
   const int k = bid_y();
   uint32_t *p = f.p[k];
   const int64_t n = f.n[k];
-  for (int64_t i = (int64_t)bid_x() * EW + tid_x(); i < n; i += (int64_t)gridDim.x * EW)
-    p[i] = f.v[k];
+  const uint32_t v = f.v[k];
+  const int64_t first = (int64_t)bid_x() * EW + tid_x(), stride = (int64_t)gridDim.x * EW;
+  int64_t done = 0;
+  if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    const int64_t n4 = n / 4;
+    uint4 *p4 = reinterpret_cast<uint4 *>(p);
+    for (int64_t i = first; i < n4; i += stride) p4[i] = make_uint4(v, v, v, v);
+    done = 4 * n4;
+  }
+  for (int64_t i = done + first; i < n; i += stride) p[i] = v;
 }
 
 // inverse == 0: dst[i][j] = src[perm[i]][perm[j]]; inverse == 1:
@@ -791,7 +810,7 @@ hipError_t launch_fill_multi(const FillSegments &f, hipStream_t s) {
   int64_t most = 0;
   for (int i = 0; i < f.k; ++i) most = std::max(most, f.n[i]);
   if (most <= 0) return hipSuccess;
-  fill_multi_kernel<<<dim3(grid_for(most), f.k), dim3(EW), 0, s>>>(f);
+  fill_multi_kernel<<<dim3(grid_for((most + 3) / 4), f.k), dim3(EW), 0, s>>>(f);
   return hipGetLastError();
 }
 
