@@ -543,10 +543,11 @@ def batched_graphs(torch, simd2_amd, count=4096, seed=0, ceiling=None):
 
 
 def learned_schedule(torch, count, n=8192):
-    """Learned-schedule APSP / MCP at n = 8192 on the first `count` held-out
-    graphs (ER / grid / geometric in turn): plain converge loop vs the learned
-    loop vs the scheduled loop without the network, every result checked bit
-    for bit against the plain loop."""
+    """Learned-schedule APSP / MCP at n (8192, or 1024 where a squaring takes
+    tens of microseconds and the features' cost shows) on the first `count`
+    held-out graphs (ER / grid / geometric in turn): plain converge loop vs
+    the learned loop vs the scheduled loop without the network, every result
+    checked bit for bit against the plain loop."""
     import os
     from simd2_amd.eval_schedule import evaluate, summarize
     from simd2_amd.learned import MODEL_PATH
@@ -646,7 +647,7 @@ def main():
                     help="held-out n=8192 graphs for the learned-schedule extra")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--only-extra", choices=["pld_knn", "batched_4096", "learned_8192",
-                                             "apsp_er1024"],
+                                             "learned_1024", "apsp_er1024"],
                     help="run one extra workload alone and print its result (profiling)")
     args = ap.parse_args()
     if args.pmc_child:
@@ -659,6 +660,7 @@ def main():
         fn = {"pld_knn": lambda: pld_knn(torch, simd2_amd),
               "batched_4096": lambda: batched_graphs(torch, simd2_amd),
               "learned_8192": lambda: learned_schedule(torch, args.learned_graphs),
+              "learned_1024": lambda: learned_schedule(torch, 200, n=1024),
               "apsp_er1024": lambda: apsp_1024(torch, simd2_amd)}[args.only_extra]
         print(json.dumps({args.only_extra: fn()}), flush=True)
         return
@@ -811,6 +813,8 @@ def main():
         torch.cuda.empty_cache()
         extras["learned_8192"] = learned_schedule(torch, args.learned_graphs)
         log(f"learned schedule n=8192: {extras['learned_8192']}")
+        extras["learned_1024"] = learned_schedule(torch, 200, n=1024)
+        log(f"learned schedule n=1024: {extras['learned_1024']}")
         torch.cuda.empty_cache()
 
     cpu = None
