@@ -78,8 +78,10 @@ size_t simd2_converge_learned_workspace(int n, int max_itr);
 
 /* The converge loop under the learned schedule: features and vertex order of
  * X, the model's prediction (one 8-byte read), then
- * simd2_converge_scheduled with that order and schedule.  model: device
- * array of SIMD2_MODEL_FLOATS.  *used (optional) receives the schedule.
+ * simd2_converge_scheduled with that order and schedule (the predicted tile
+ * shape up to n = 4096, the sizes the shipped model was trained on; the
+ * automatic tile choice above).  model: device array of
+ * SIMD2_MODEL_FLOATS.  *used (optional) receives the schedule.
  * Returns like simd2_converge_f32. */
 int simd2_converge_learned(int op, int n, float *X, int ldx, int do_epilogue, int check_mode,
                            int max_itr, const float *model, void *workspace,

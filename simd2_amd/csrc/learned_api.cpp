@@ -59,6 +59,11 @@ int order_into(int n, const float *X, int ldx, int *perm_dev, void *ws, bool aft
 }
 
 constexpr int CFG_OF_CLASS[3] = {CFG_S64, CFG_M128x64, CFG_L128};  // by tile size
+// The shipped model's training graphs have n in [256, 4096].  Above that its
+// tile class is an extrapolation (at n = 8192 it chose 64x64 tiles for grids,
+// 4% slower than the size rule's 128x128), so larger graphs take the size
+// rule's tile and only the squaring count from the network.
+constexpr int TILE_TRAINED_MAX_N = 4096;
 
 }  // namespace
 
@@ -175,7 +180,9 @@ int simd2_converge_learned(int op, int n, float *X, int ldx, int do_epilogue, in
   const float itr_pred = std::isfinite(host[0]) ? host[0] : 1.0f;
   sc.predicted_itr = std::max(1, std::min(max_itr, (int)std::ceil(itr_pred + 0.25f)));
   const float cls = std::isfinite(host[1]) ? host[1] : 2.0f;
-  sc.tile_config = CFG_OF_CLASS[std::max(0, std::min(2, (int)std::lround(cls)))];
+  sc.tile_config = n > TILE_TRAINED_MAX_N
+                       ? valu_pick_cfg(op, n, n, n)
+                       : CFG_OF_CLASS[std::max(0, std::min(2, (int)std::lround(cls)))];
   // Vertex order: breadth-first levels when the given numbering has no
   // locality (mean |i - j| over edges above n / 20), one search reaches most
   // of the graph, and the graph is "long" (mean search depth above log2 n): a
