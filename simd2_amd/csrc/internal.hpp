@@ -35,6 +35,7 @@ enum ValuCfg { CFG_AUTO = -1, CFG_L128 = 0, CFG_S64 = 1, CFG_M128x64 = 2, CFG_CO
 struct ValuCfgInfo { int bm, bn, bk, threads; };
 ValuCfgInfo valu_cfg_info(int cfg);
 int valu_pick_cfg(int op, int R, int Cn, int K);
+int device_cu_count();  // compute units of the current device (first call cached)
 void valu_record_cfg(int op, int R, int Cn, int K, int cfg);
 
 hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,

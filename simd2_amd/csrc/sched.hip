@@ -725,19 +725,6 @@ __global__ __launch_bounds__(EW) void permute_copy_kernel(float *dst, int64_t ld
 
 int grid_for(int64_t n) { return (int)std::min<int64_t>((n + EW - 1) / EW, 4096); }
 
-int device_cus() {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0, n = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
-      cus = n;
-    else
-      cus = 256;
-  }
-  return cus;
-}
-
 template <class Op>
 hipError_t sched_op(int cfg, const SchedArgs &a, hipStream_t s) {
   const dim3 grid(a.tiles_r * a.tiles_c), block(256);
@@ -758,7 +745,7 @@ hipError_t sched_op(int cfg, const SchedArgs &a, hipStream_t s) {
     }
   }
   // at most four 64x64 tiles per CU: the tile's slabs split over 4 or 2 groups
-  const int tiles = a.tiles_r * a.tiles_c, cus = device_cus();
+  const int tiles = a.tiles_r * a.tiles_c, cus = device_cu_count();
   const bool split = cfg == CFG_S64 && a.slabs <= SCHED_SPLIT_SLABS;
   if (split && tiles < 2 * cus)
     sched_kernel<Op, CfgS, 4><<<grid, dim3(1024), 0, s>>>(a);

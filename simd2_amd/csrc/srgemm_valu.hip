@@ -161,6 +161,8 @@ hipError_t launch_op(int R, int Cn, int K, const float *X, int64_t ldx,
   return launch_cfg<Op, CfgL>(R, Cn, K, X, ldx, Y, ldy, O, ldo, e, s);
 }
 
+}  // namespace
+
 int device_cu_count() {
   static int cus = 0;
   if (cus == 0) {
@@ -173,8 +175,6 @@ int device_cu_count() {
   }
   return cus;
 }
-
-}  // namespace
 
 ValuCfgInfo valu_cfg_info(int cfg) {
   if (cfg == CFG_S64) return {CfgS::BM, CfgS::BN, CfgS::BK, 256};

@@ -183,6 +183,25 @@ def test_plus_mul_mfma_every_shape(gpu, M, N, K, pad):
         assert np.all(D[:, N:] == 7.0)  # padding untouched
 
 
+def test_plus_mul_tile_sizes_identical(gpu):
+    """The MFMA kernel's 64x64 tiles (problems with fewer 128x128 tiles than
+    CUs) and 128x128 tiles sum every element over the same k order: rows of a
+    2048 x 2048 product (128x128 tiles) equal the same rows computed as a
+    256-row problem (64x64 tiles), bit for bit."""
+    import simd2_amd
+    torch = gpu
+    g = torch.Generator(device="cuda")
+    g.manual_seed(31)
+    A = torch.randn((2048, 1000), generator=g, device="cuda")
+    B = torch.randn((1000, 2048), generator=g, device="cuda")
+    C = torch.randn((2048, 2048), generator=g, device="cuda")
+    say("plus_mul 2048^2 x 1000 in both tile sizes")
+    full = simd2_amd.srgemm("plus_mul", A, B, C, True)
+    part = simd2_amd.srgemm("plus_mul", A[512:768].contiguous(), B, C[512:768].contiguous(), True)
+    torch.cuda.synchronize()
+    assert torch.equal(full[512:768].view(torch.int32), part.view(torch.int32))
+
+
 def test_large_rows_sampled(gpu, oracle_mod):
     """min_plus and max_min at 2048^3: rows sampled against the oracle."""
     import torch
