@@ -66,6 +66,9 @@ enum {
   SIMD2_ERROR_NOT_CONVERGED = 14,
   SIMD2_ERROR_WORKSPACE = 15       /* workspace missing or too small */
 };
+/* Every workspace argument of this library is device memory whose base is
+ * 256-byte aligned (hipMalloc's and PyTorch's allocations are); a misaligned
+ * one is refused with SIMD2_ERROR_MISALIGNED_OPERAND. */
 
 /* Human-readable name of a status code (static storage). */
 const char *simd2_status_string(int status);

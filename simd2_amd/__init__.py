@@ -46,6 +46,7 @@ DROPIN = {
 
 # Status codes (include/simd2/srgemm.h).
 SUCCESS = 0
+ERROR_MISALIGNED_OPERAND = 1
 ERROR_INVALID_PROBLEM = 4
 ERROR_NOT_SUPPORTED = 5
 ERROR_INTERNAL = 7

@@ -239,6 +239,7 @@ int simd2_minus_square_exact(int M, int N, int K, const float *A, int lda, const
     if (overlap(w, span(A, M, K, lda, 4)) || overlap(w, span(B, K, N, ldb, 4)) ||
         overlap(w, span(D, M, N, ldd, 4)) || (epi && overlap(w, span(C, M, N, ldc, 4))))
       return SIMD2_ERROR_ALIASING;
+    if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
   }
   return hip_status(launch_minus_square_exact(N, M, K, B, ldb, A, lda, D, ldd,
                                               epi ? C : nullptr, ldc, workspace,

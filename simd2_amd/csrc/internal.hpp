@@ -36,6 +36,11 @@ struct ValuCfgInfo { int bm, bn, bk, threads; };
 ValuCfgInfo valu_cfg_info(int cfg);
 int valu_pick_cfg(int op, int R, int Cn, int K);
 int device_cu_count();  // compute units of the current device (first call cached)
+// Caller workspaces are carved into 256-byte aligned pieces that the kernels
+// read with 16-byte vector loads: their base must be 256-byte aligned.
+inline bool workspace_aligned(const void *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 255) == 0;
+}
 void valu_record_cfg(int op, int R, int Cn, int K, int cfg);
 
 hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,

@@ -31,6 +31,7 @@ int simd2_knn_select(const float *dist, int query_nb, int ref_nb, int k, float *
   if (!dist || !knn_dist || !knn_index) return SIMD2_ERROR_NULL_POINTER;
   if (!workspace || workspace_bytes < knn_select_workspace(query_nb, ref_nb, k))
     return SIMD2_ERROR_WORKSPACE;
+  if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
   return status_of(launch_knn_select(dist, query_nb, ref_nb, k, knn_dist, knn_index,
                                      (flags & SIMD2_KNN_SQRT) ? 1 : 0, workspace, stream));
 }
@@ -50,6 +51,7 @@ int simd2_knn_bf16(const float *query, int ldq, const float *ref, int ldr, int q
   if (!query || !ref || !knn_dist || !knn_index) return SIMD2_ERROR_NULL_POINTER;
   if (!workspace || workspace_bytes < knn_bf16_workspace(query_nb, ref_nb, dim))
     return SIMD2_ERROR_WORKSPACE;
+  if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
   return status_of(launch_knn_bf16(query, ldq, ref, ldr, query_nb, ref_nb, dim, k, knn_dist,
                                    knn_index, (flags & SIMD2_KNN_SQRT) ? 1 : 0, workspace, stats,
                                    stream));

@@ -80,6 +80,7 @@ int simd2_graph_features(int op, int n, const float *X, int ldx, int do_epilogue
   if (n <= 0 || n > MAX_N || ldx < n) return SIMD2_ERROR_INVALID_PROBLEM;
   if (!X || !features || !workspace) return SIMD2_ERROR_NULL_POINTER;
   if (workspace_bytes < simd2_graph_features_workspace(n)) return SIMD2_ERROR_WORKSPACE;
+  if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
   return hip_status(launch_graph_features(op, n, X, ldx, do_epilogue, features, workspace, stream));
 }
 
@@ -89,6 +90,7 @@ int simd2_graph_order(int op, int n, const float *X, int ldx, int *perm, void *w
   if (n <= 0 || n > MAX_N || ldx < n) return SIMD2_ERROR_INVALID_PROBLEM;
   if (!X || !perm || !workspace) return SIMD2_ERROR_NULL_POINTER;
   if (workspace_bytes < simd2_graph_features_workspace(n)) return SIMD2_ERROR_WORKSPACE;
+  if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
   return order_into(n, X, ldx, perm, workspace, false, stream, nullptr);
 }
 
@@ -110,6 +112,7 @@ int simd2_mlp_gradient(const float *params, const float *x, const float *t, cons
   if (B <= 0) return SIMD2_ERROR_INVALID_PROBLEM;
   if (!params || !x || !t || !w || !workspace) return SIMD2_ERROR_NULL_POINTER;
   if (workspace_bytes < simd2_mlp_workspace(B)) return SIMD2_ERROR_WORKSPACE;
+  if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
   return hip_status(launch_mlp_gradient(params, x, t, w, B, grad, loss, workspace, stream));
 }
 
@@ -119,6 +122,7 @@ int simd2_mlp_train_step(float *params, float *m, float *v, int step, float lr, 
   if (B <= 0 || step < 1) return SIMD2_ERROR_INVALID_PROBLEM;
   if (!params || !m || !v || !x || !t || !w || !workspace) return SIMD2_ERROR_NULL_POINTER;
   if (workspace_bytes < simd2_mlp_workspace(B)) return SIMD2_ERROR_WORKSPACE;
+  if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
   float *grad = reinterpret_cast<float *>(static_cast<char *>(workspace) +
                                           align256(mlp_workspace(B)));
   hipError_t e = launch_mlp_gradient(params, x, t, w, B, grad, loss, workspace, stream);
@@ -150,6 +154,7 @@ int simd2_converge_learned(int op, int n, float *X, int ldx, int do_epilogue, in
   if (!X || !model || !workspace) return SIMD2_ERROR_NULL_POINTER;
   if (workspace_bytes < simd2_converge_learned_workspace(n, max_itr))
     return SIMD2_ERROR_WORKSPACE;
+  if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
   char *ws = static_cast<char *>(workspace);
   const size_t fbytes = simd2_graph_features_workspace(n);
   float *feat = reinterpret_cast<float *>(ws + fbytes);

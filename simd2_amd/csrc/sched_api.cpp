@@ -137,6 +137,7 @@ int simd2_converge_scheduled(int op, int n, float *X, int ldx, int do_epilogue,
   Layout L;
   if (!layout(n, cfg, max_itr, &L)) return SIMD2_ERROR_INVALID_PROBLEM;
   if (workspace_bytes < L.total) return SIMD2_ERROR_WORKSPACE;
+  if (!workspace_aligned(workspace)) return SIMD2_ERROR_MISALIGNED_OPERAND;
 
   char *ws = static_cast<char *>(workspace);
   float *buf[2] = {reinterpret_cast<float *>(ws + L.buf[0]),

@@ -212,6 +212,13 @@ def test_workspace_checks_and_reuse(gpu, oracle_mod):
                                                   big_out.data_ptr() + 4 * N * 2, need,
                                                   torch.cuda.current_stream().cuda_stream)
     assert st == simd2_amd.ERROR_ALIASING
+    # a workspace not 256-byte aligned
+    spare = torch.empty(need + 256, dtype=torch.uint8, device="cuda")
+    st = simd2_amd.lib().simd2_minus_square_exact(N, M, K, tB.data_ptr(), N, tA.data_ptr(), K,
+                                                  tC.data_ptr(), N, out.data_ptr(), N, 1,
+                                                  spare.data_ptr() + 16, need,
+                                                  torch.cuda.current_stream().cuda_stream)
+    assert st == simd2_amd.ERROR_MISALIGNED_OPERAND
     torch.cuda.synchronize()
     assert bool((out == 7.0).all()) and bool((big_out == 7.0).all())
     big = torch.empty(need, dtype=torch.uint8, device="cuda")

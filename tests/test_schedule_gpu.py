@@ -124,6 +124,13 @@ def test_cap_and_edge_sizes(gpu):
     Z, itr2, st2, _ = converge_scheduled("min_plus", path.clone(), max_itr=4, predicted_itr=2)
     assert st == st2 == simd2_amd.ERROR_NOT_CONVERGED and itr == itr2 == 4
     assert torch.equal(Y, Z)
+    # a workspace whose base is not 256-byte aligned is refused, X untouched
+    need = simd2_amd.lib().simd2_converge_scheduled_workspace(n, -1, 4)
+    ws = torch.empty(need + 256, dtype=torch.uint8, device="cuda")
+    with pytest.raises(simd2_amd.Simd2Error) as err:
+        converge_scheduled("min_plus", path, max_itr=4, workspace=ws[64:])
+    assert err.value.status == simd2_amd.ERROR_MISALIGNED_OPERAND
+    assert torch.equal(path[0, :3].cpu(), torch.tensor([0.0, 1.0, float("inf")]))
     for n in (1, 2, 15, 16, 17, 129):
         X = er_family(torch, "min_plus", n, 0.3, n)
         Y, itr, _ = plain(torch, "min_plus", X, True, "threshold")
