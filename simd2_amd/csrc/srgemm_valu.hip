@@ -65,8 +65,11 @@ __attribute__((amdgpu_waves_per_eu(WAVES))) void srgemm_valu_kernel(
 // slabs with its own LDS buffers, and the partial tiles are combined in
 // group order, ((p0 (+) p1) (+) p2) (+) p3, before the usual epilogue: G
 // times the resident waves.  Only for the semirings whose (+) is min or max,
-// which is exact: regrouping the terms changes no bit.
-template <class Op, int SK_GROUPS>
+// which is exact: regrouping the terms changes no bit.  CHECKED: any shape
+// and alignment, per-element bounds on loads and stores; every group runs
+// ceil(slabs / G) slabs, the ranges past K reading padding, whose terms are
+// the (+)-identity.
+template <class Op, int SK_GROUPS, bool CHECKED>
 __global__ __launch_bounds__(256 * SK_GROUPS) void srgemm_valu_splitk_kernel(
     int R, int Cn, int K, const float *__restrict__ X, int64_t ldx,
     const float *__restrict__ Y, int64_t ldy, float *O, int64_t ldo, Epilogue e,
@@ -79,9 +82,9 @@ __global__ __launch_bounds__(256 * SK_GROUPS) void srgemm_valu_splitk_kernel(
   const int row0 = tr * S::BM, col0 = tc * S::BN;
   const int g = tid_x() / 256, t = tid_x() % 256;
   if (tid_x() == 0) sm[0].changed = 0;  // published by the main loop's barriers
-  const int per = K / S::BK / SK_GROUPS;
+  const int per = ((K + S::BK - 1) / S::BK + SK_GROUPS - 1) / SK_GROUPS;
   float acc[S::TM][S::TN];
-  valu_mainloop<Op, S::BM, S::BN, S::TM, S::TN, S::BK, false, false>(
+  valu_mainloop<Op, S::BM, S::BN, S::TM, S::TN, S::BK, CHECKED, false>(
       R, Cn, K, X, ldx, Y, ldy, row0, col0, nullptr, 0, sm[g], acc, t, g * per, (g + 1) * per);
   // The main loop ends with a barrier after every read of the slabs: groups
   // 1.. park their partial tiles in their own LDS (16 KB of its 16.5).
@@ -106,20 +109,30 @@ __global__ __launch_bounds__(256 * SK_GROUPS) void srgemm_valu_splitk_kernel(
           acc[i][j] = Op::add(acc[i][j], part[(i * S::TN + j) * 256 + t]);
     }
   }
-  valu_epilogue<Op, S::BM, S::BN, S::TM, S::TN, false, false>(R, Cn, O, ldo, e, row0, col0, flag,
-                                                              &sm[0].changed, acc, t, g == 0);
+  valu_epilogue<Op, S::BM, S::BN, S::TM, S::TN, CHECKED, false>(R, Cn, O, ldo, e, row0, col0,
+                                                                flag, &sm[0].changed, acc, t,
+                                                                g == 0);
 }
 
+// groups > 0: the fast form (whole tiles and slab groups, aligned rows);
+// groups < 0: the checked form with -groups groups.
 template <class Op>
 hipError_t launch_splitk(int R, int Cn, int K, const float *X, int64_t ldx, const float *Y,
                          int64_t ldy, float *O, int64_t ldo, const Epilogue &e, int groups,
                          hipStream_t s) {
-  const int tiles_r = R / CfgS::BM, tiles_c = Cn / CfgS::BN;
+  const int tiles_r = (R + CfgS::BM - 1) / CfgS::BM, tiles_c = (Cn + CfgS::BN - 1) / CfgS::BN;
+  const dim3 grid(tiles_r * tiles_c);
   if (groups == 4)
-    srgemm_valu_splitk_kernel<Op, 4><<<dim3(tiles_r * tiles_c), dim3(1024), 0, s>>>(
+    srgemm_valu_splitk_kernel<Op, 4, false><<<grid, dim3(1024), 0, s>>>(
+        R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
+  else if (groups == 2)
+    srgemm_valu_splitk_kernel<Op, 2, false><<<grid, dim3(512), 0, s>>>(
+        R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
+  else if (groups == -4)
+    srgemm_valu_splitk_kernel<Op, 4, true><<<grid, dim3(1024), 0, s>>>(
         R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
   else
-    srgemm_valu_splitk_kernel<Op, 2><<<dim3(tiles_r * tiles_c), dim3(512), 0, s>>>(
+    srgemm_valu_splitk_kernel<Op, 2, true><<<grid, dim3(512), 0, s>>>(
         R, Cn, K, X, ldx, Y, ldy, O, ldo, e, tiles_r, tiles_c);
   return hipGetLastError();
 }
@@ -227,23 +240,29 @@ int valu_pick_cfg(int op, int R, int Cn, int K) {
 }
 
 // The split-K form (automatic choice only): a min/max semiring, the 64x64
-// tiles chosen, a whole number of tiles and of G-slab groups, every operand
-// 16-byte aligned row by row, at most four tiles per CU, no gate.  Returns
-// the group count G, 0 when the form does not apply.
+// tiles chosen, at most four tiles per CU, no gate.  Returns the group count
+// G for the fast form (a whole number of tiles and of G-slab groups, every
+// operand 16-byte aligned row by row), -G for the checked form (operands
+// whose rows are not all 16-byte aligned, where the unsplit kernel takes its
+// checked body too, and aligned off-grid shapes of at most two tiles per CU,
+// where four groups' waves outweigh the unsplit kernel's 128-bit interior
+// loads: 1000x776x300 min_plus 39 -> 31 us), 0 when the split does not apply.
 static int splitk_groups(int op, int R, int Cn, int K, const float *X, int64_t ldx,
                          const float *Y, int64_t ldy, const float *O, int64_t ldo,
                          const Epilogue &e) {
-  if (op == 1 || op == 8 || op < 0 || op > 6 || e.gate) return 0;
-  if (R % CfgS::BM || Cn % CfgS::BN || K <= 0) return 0;
-  const int64_t tiles = (int64_t)(R / CfgS::BM) * (Cn / CfgS::BN);
+  if (op == 1 || op == 8 || op < 0 || op > 6 || e.gate || K <= 0) return 0;
+  const int64_t tiles =
+      (int64_t)((R + CfgS::BM - 1) / CfgS::BM) * ((Cn + CfgS::BN - 1) / CfgS::BN);
   const int groups = tiles < 2 * device_cu_count() ? 4 : tiles <= 4 * device_cu_count() ? 2 : 0;
-  if (groups == 0 || K % (CfgS::BK * groups)) return 0;
+  if (groups == 0) return 0;
   auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
   bool vec = ldx % 4 == 0 && ldy % 4 == 0 && ldo % 4 == 0 && aligned(X) && aligned(Y) &&
              aligned(O);
   if (e.C) vec = vec && e.ldc % 4 == 0 && aligned(e.C);
   if (e.P) vec = vec && e.ldp % 4 == 0 && aligned(e.P);
-  return vec ? groups : 0;
+  if (!vec) return -groups;
+  if (R % CfgS::BM == 0 && Cn % CfgS::BN == 0 && K % (CfgS::BK * groups) == 0) return groups;
+  return groups == 4 ? -4 : 0;
 }
 
 hipError_t launch_valu_f32(int op, int R, int Cn, int K, const float *X,

@@ -351,14 +351,17 @@ def test_autotune_records_winner(gpu, oracle_mod):
 
 
 @pytest.mark.parametrize("M,N,K", [(1024, 1024, 1024), (512, 768, 64), (256, 256, 4096),
-                                   (1536, 1536, 256), (2048, 2048, 64)])
+                                   (1536, 1536, 256), (2048, 2048, 64), (1000, 777, 300),
+                                   (1000, 776, 300), (130, 70, 65), (33, 50, 17)])
 def test_small_problems_split_k(gpu, oracle_mod, M, N, K):
     """Problems with at most four 64x64 tiles per CU take the split-K form
     (four groups over contiguous slab ranges up to two tiles per CU, two
     groups up to four: 1536^2 and 2048^2; partials combined in order) for
-    the six min/max semirings: the oracle's bits with and without C, and the
-    same bits as the unsplit 64x64 kernel (tile_config 1) on data with signed
-    zeros, whose ties only an order-independent (+) leaves alone."""
+    the six min/max semirings -- the checked form when rows are not 16-byte
+    aligned (N = 777, 70, 50) or the shape is off the tile grid (1000 x 776):
+    edge tiles, K past the last whole slab, group ranges past K --: the oracle's bits with and without C, and the same bits
+    as the unsplit 64x64 kernel (tile_config 1) on data with signed zeros,
+    whose ties only an order-independent (+) leaves alone."""
     import simd2_amd
     for op in EXACT_OPS:
         for epi in (False, True):
